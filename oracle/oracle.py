@@ -1,0 +1,177 @@
+"""Python face of the CPU oracle (oracle.c) plus the seeded input generators.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker. The shipped library never
+imports it. Every function names the reference code it restates; oracle.c
+holds the algorithms, this file only marshals numpy arrays.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+# clc_type codes (reference lib/include/CL/Compute/compute_c.h:44-56).
+TYPE_CODES = {
+    np.dtype(np.int8): 0, np.dtype(np.uint8): 1,
+    np.dtype(np.int16): 2, np.dtype(np.uint16): 3,
+    np.dtype(np.int32): 4, np.dtype(np.uint32): 5,
+    np.dtype(np.int64): 6, np.dtype(np.uint64): 7,
+    np.dtype(np.float32): 8, np.dtype(np.float64): 9,
+}
+
+# Operator codes of oracle.c: the canonical ops:: strings of the reference
+# (lib/include/CL/Compute/Algorithm.hpp:53-65) and the projections "b"/"a".
+OPS = {"plus": 0, "multiplies": 1, "bit_and": 2, "bit_or": 3, "bit_xor": 4,
+       "minimum": 5, "maximum": 6, "right": 7, "left": 8}
+
+_lib = None
+
+
+def build() -> str:
+    """Compile liboracle.so with the Makefile beside this file."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        vp, sz, u64 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint64
+        L.oracle_pcg32_u32.argtypes = [u64, u64, u64, vp, sz]
+        L.oracle_pcg32_u32.restype = None
+        L.oracle_ints_range.argtypes = [u64, u64, vp, sz, ctypes.c_int32,
+                                        ctypes.c_int32]
+        L.oracle_ints_range.restype = None
+        L.oracle_radix_sort.argtypes = [ctypes.c_int, vp, vp, sz, ctypes.c_int,
+                                        ctypes.c_int]
+        L.oracle_radix_sort.restype = ctypes.c_int
+        L.oracle_scan.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp, vp, sz,
+                                  ctypes.c_int]
+        L.oracle_scan.restype = ctypes.c_int
+        L.oracle_reduce.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp, vp, sz,
+                                    vp]
+        L.oracle_reduce.restype = ctypes.c_int
+        L.oracle_histogram_even.argtypes = [ctypes.c_int, vp, sz, vp, sz,
+                                            ctypes.c_double, ctypes.c_double]
+        L.oracle_histogram_even.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+# ---- generators (reference lib/src/SDK/Random.c) -------------------------
+
+def pcg32_u32(initstate: int, initseq: int, n: int, skip: int = 0) -> np.ndarray:
+    """n raw pcg32_random_r outputs after pcg32_srandom_r(initstate, initseq)
+    (Random.c:13-22, :43-50), starting `skip` draws into the stream."""
+    out = np.empty(n, dtype=np.uint32)
+    lib().oracle_pcg32_u32(initstate, initseq, skip, _ptr(out), n)
+    return out
+
+
+def ints_range(initstate: int, initseq: int, n: int, low: int, hi: int) -> np.ndarray:
+    """cl_sdk_fill_with_random_ints_range (Random.c:73-102)."""
+    out = np.empty(n, dtype=np.int32)
+    lib().oracle_ints_range(initstate, initseq, _ptr(out), n, low, hi)
+    return out
+
+
+def pcg32_u64(initstate: int, initseq: int, n: int, bits: int = 64) -> np.ndarray:
+    """64-bit keys from two pcg32 draws, (hi << 32 | lo), masked to `bits`."""
+    raw = pcg32_u32(initstate, initseq, 2 * n).astype(np.uint64)
+    k = (raw[0::2] << np.uint64(32)) | raw[1::2]
+    if bits < 64:
+        k &= np.uint64((1 << bits) - 1)
+    return k
+
+
+# ---- key transform (reference lib/src/Compute/Algorithm.cpp:459-469) -------
+
+def transformed_keys(keys: np.ndarray, descending: bool = False) -> np.ndarray:
+    """The unsigned key whose ascending order the reference sort follows:
+    uint k; int k ^ signbit; float (k & sign) ? ~k : k | sign; descending ~.
+    Extended to 64-bit keys with the same formulas."""
+    dt = keys.dtype
+    if dt.itemsize == 4:
+        u = keys.view(np.uint32).copy()
+        top = np.uint32(0x80000000)
+    elif dt.itemsize == 8:
+        u = keys.view(np.uint64).copy()
+        top = np.uint64(0x8000000000000000)
+    else:
+        raise ValueError("keys must be 32- or 64-bit")
+    if dt.kind == "i":
+        u ^= top
+    elif dt.kind == "f":
+        neg = (u & top) != 0
+        u = np.where(neg, ~u, u | top)
+    if descending:
+        u = ~u
+    return u
+
+
+# ---- algorithms -----------------------------------------------------------
+
+def radix_sort(keys: np.ndarray, values: np.ndarray | None = None,
+               descending: bool = False, threads: int = 1):
+    """Stable LSD radix sort restated from Algorithm.cpp:129-213, :453-527.
+    Returns sorted copies (keys, values-or-None)."""
+    k = np.ascontiguousarray(keys).copy()
+    v = None if values is None else np.ascontiguousarray(values, dtype=np.uint32).copy()
+    code = TYPE_CODES[k.dtype]
+    rc = lib().oracle_radix_sort(code, _ptr(k), _ptr(v), k.size,
+                                 1 if descending else 0, threads)
+    if rc == -1:
+        raise ValueError("oracle_radix_sort: key type must be 32- or 64-bit")
+    if rc != 0:
+        raise MemoryError("oracle_radix_sort: host allocation failed")
+    return k, v
+
+
+def scan(x: np.ndarray, op: str = "plus", identity=0, exclusive: bool = False) -> np.ndarray:
+    """Sequential left-to-right scan (Algorithm.cpp:76-127, :280-318)."""
+    x = np.ascontiguousarray(x)
+    out = np.empty_like(x)
+    ident = np.array([identity], dtype=x.dtype)
+    rc = lib().oracle_scan(TYPE_CODES[x.dtype], OPS[op], _ptr(ident), _ptr(x),
+                           _ptr(out), x.size, 1 if exclusive else 0)
+    if rc != 0:
+        raise ValueError(f"oracle_scan: operator {op} undefined for {x.dtype}")
+    return out
+
+
+def reduce(x: np.ndarray, op: str = "plus", identity=0, init=0):
+    """op(init, fold(identity, x)) (Algorithm.cpp:58-74, :400-442)."""
+    x = np.ascontiguousarray(x)
+    ident = np.array([identity], dtype=x.dtype)
+    ini = np.array([init], dtype=x.dtype)
+    res = np.zeros(1, dtype=x.dtype)
+    rc = lib().oracle_reduce(TYPE_CODES[x.dtype], OPS[op], _ptr(ident),
+                             _ptr(ini), _ptr(x), x.size, _ptr(res))
+    if rc != 0:
+        raise ValueError(f"oracle_reduce: operator {op} undefined for {x.dtype}")
+    return res[0]
+
+
+def histogram_even(x: np.ndarray, bins: int, lower: float, upper: float) -> np.ndarray:
+    """Even-width histogram with the reference's float binning
+    (Algorithm.cpp:240-273, :569-615)."""
+    x = np.ascontiguousarray(x)
+    h = np.zeros(max(bins, 1), dtype=np.uint32)
+    rc = lib().oracle_histogram_even(TYPE_CODES[x.dtype], _ptr(x), x.size,
+                                     _ptr(h), bins, lower, upper)
+    if rc != 0:
+        raise ValueError("oracle_histogram_even: lower must be below upper")
+    return h[:bins]

@@ -1,0 +1,214 @@
+"""Pins the CPU oracle before anything is checked against it.
+
+Each oracle function is checked against something that does not restate
+its own formula: the published pcg32 demo sequence, the reference tests'
+own inputs and std::stable_sort / std::sort results (tests/golden), numpy's
+stable argsort, closed forms (numpy cumsum/sum with uint32 wrap), brute
+force on small inputs, and planted mistakes that the checks must catch.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+SIZES = [0, 1, 7, 255, 256, 1000, 4097]
+
+
+# ---- pcg32 -------------------------------------------------------------------
+
+def test_pcg32_published_demo_sequence():
+    # pcg32-demo (O'Neill, pcg-c-basic) prints these for srandom(42, 54).
+    got = oracle.pcg32_u32(42, 54, 6)
+    assert [int(x) for x in got] == [0xa15c02b7, 0x7b47f409, 0xba1d3330,
+                                     0x83d2f293, 0xbfa4784b, 0xcbed606e]
+
+
+def test_pcg32_advance_matches_sequential():
+    full = oracle.pcg32_u32(7, 11, 5000)
+    for skip in (0, 1, 63, 1000, 4999):
+        part = oracle.pcg32_u32(7, 11, 5000 - skip, skip=skip)
+        np.testing.assert_array_equal(part, full[skip:])
+
+
+def test_ints_range_bounds_and_byte_mask():
+    v = oracle.ints_range(42, 54, 20000, 0, 255)
+    assert v.min() >= 0 and v.max() <= 255
+    # diff = 255 needs 8 bits and never rejects: the low byte of each draw.
+    raw = oracle.pcg32_u32(42, 54, 20000)
+    np.testing.assert_array_equal(v, (raw & 0xFF).astype(np.int32))
+    w = oracle.ints_range(1, 2, 20000, -5, 9)
+    assert w.min() == -5 and w.max() == 9
+
+
+# ---- radix sort --------------------------------------------------------------
+
+@pytest.mark.parametrize("n", SIZES)
+def test_sort_reference_test_vectors(golden, n):
+    keys = golden[f"sort_keys_{n}"]
+    idx = np.arange(n, dtype=np.uint32)
+    k, v = oracle.radix_sort(keys, idx)
+    np.testing.assert_array_equal(k, golden[f"sort_expect_keys_{n}"])
+    np.testing.assert_array_equal(v, golden[f"sort_expect_values_{n}"])
+
+
+def test_sort_stability_reference_vector(golden):
+    keys = golden["stable_keys_u8"].astype(np.uint32)
+    k, v = oracle.radix_sort(keys, np.arange(keys.size, dtype=np.uint32))
+    np.testing.assert_array_equal(v, golden["stable_expect_values"])
+    np.testing.assert_array_equal(k, keys[golden["stable_expect_values"]])
+
+
+def test_sort_signed_reference_vector(golden):
+    k, _ = oracle.radix_sort(golden["ints_in"])
+    np.testing.assert_array_equal(k, golden["ints_expect"])
+
+
+def test_sort_float_descending_reference_vector(golden):
+    f = golden["floats_in"]
+    k, _ = oracle.radix_sort(f, descending=True)
+    # The reference compares with ==, so -0 and +0 are alike (main.cpp:309).
+    assert np.array_equal(k, golden["floats_expect_desc"])
+
+
+@pytest.mark.parametrize("dtype", [np.uint32, np.int32, np.float32,
+                                   np.uint64, np.int64, np.float64])
+@pytest.mark.parametrize("descending", [False, True])
+@pytest.mark.parametrize("threads", [1, 3])
+def test_sort_matches_numpy_stable_argsort(dtype, descending, threads):
+    rng = np.random.default_rng(1234)
+    n = 20011
+    if np.dtype(dtype).kind == "f":
+        keys = rng.standard_normal(n).astype(dtype) * 1e3
+        keys[:50] = 0.0
+        keys[50:100] = -0.0
+        keys[100:110] = np.inf
+        keys[110:120] = -np.inf
+        keys[200:2200] = keys[3000]  # duplicates
+    else:
+        info = np.iinfo(dtype)
+        keys = rng.integers(info.min, info.max, size=n, dtype=dtype, endpoint=True)
+        keys[:300] = keys[400]
+        keys[300:310] = info.min
+        keys[310:320] = info.max
+    vals = rng.integers(0, 2**32, size=n, dtype=np.uint32)
+    perm = np.argsort(oracle.transformed_keys(keys, descending), kind="stable")
+    k, v = oracle.radix_sort(keys, vals, descending=descending, threads=threads)
+    np.testing.assert_array_equal(k.view(np.uint8), keys[perm].view(np.uint8))
+    np.testing.assert_array_equal(v, vals[perm])
+
+
+def test_sort_negative_zero_before_positive_zero():
+    keys = np.array([0.0, -0.0, 1.0, -0.0, 0.0, -1.0], dtype=np.float32)
+    k, v = oracle.radix_sort(keys, np.arange(6, dtype=np.uint32))
+    assert v.tolist() == [5, 1, 3, 0, 4, 2]
+    k, v = oracle.radix_sort(keys, np.arange(6, dtype=np.uint32), descending=True)
+    assert v.tolist() == [2, 0, 4, 1, 3, 5]
+
+
+def test_sort_small_brute_force():
+    rng = np.random.default_rng(5)
+    for n in range(0, 40):
+        keys = rng.integers(0, 4, size=n, dtype=np.uint32) << np.uint32(29)
+        vals = np.arange(n, dtype=np.uint32)
+        pairs = sorted(zip(keys.tolist(), vals.tolist()), key=lambda p: p[0])
+        k, v = oracle.radix_sort(keys, vals)
+        assert list(zip(k.tolist(), v.tolist())) == pairs
+
+
+def test_transform_mistakes_are_caught():
+    """A planted sign mistake in the float transform must break the
+    numpy comparison used above."""
+    keys = np.array([-2.0, -1.0, 0.5, 3.0], dtype=np.float32)
+    good = np.argsort(oracle.transformed_keys(keys), kind="stable")
+    u = keys.view(np.uint32)
+    bad_t = np.where((u & 0x80000000) != 0, u | 0x80000000, u)  # no inversion
+    bad = np.argsort(bad_t, kind="stable")
+    assert good.tolist() == [0, 1, 2, 3]
+    assert bad.tolist() != good.tolist()
+
+
+# ---- scan / reduce -----------------------------------------------------------
+
+@pytest.mark.parametrize("n", SIZES)
+def test_scan_reference_test_vectors(golden, n):
+    x = golden[f"scan_in_{n}"]
+    np.testing.assert_array_equal(oracle.scan(x, "plus", 0, False),
+                                  golden[f"scan_expect_inclusive_{n}"])
+    np.testing.assert_array_equal(oracle.scan(x, "plus", 0, True),
+                                  golden[f"scan_expect_exclusive_{n}"])
+
+
+def test_scan_projection_and_max_reference_vectors(golden):
+    v = golden["proj_in"]
+    np.testing.assert_array_equal(oracle.scan(v, "right", 0), v)
+    np.testing.assert_array_equal(oracle.scan(v, "maximum", np.iinfo(np.int32).min),
+                                  golden["max_scan_expect"])
+
+
+def test_scan_closed_form_with_wrap():
+    rng = np.random.default_rng(3)
+    x = rng.integers(0, 2**32, size=100000, dtype=np.uint32)
+    inc = np.cumsum(x, dtype=np.uint64).astype(np.uint32)  # mod 2^32
+    np.testing.assert_array_equal(oracle.scan(x, "plus", 0), inc)
+    exc = np.concatenate([[0], inc[:-1]]).astype(np.uint32)
+    np.testing.assert_array_equal(oracle.scan(x, "plus", 0, exclusive=True), exc)
+    np.testing.assert_array_equal(oracle.scan(x, "bit_xor", 0),
+                                  np.bitwise_xor.accumulate(x))
+    np.testing.assert_array_equal(oracle.scan(x, "minimum", 0xFFFFFFFF),
+                                  np.minimum.accumulate(x))
+
+
+def test_scan_exclusive_starts_at_identity():
+    x = np.arange(1, 6, dtype=np.int32)
+    assert oracle.scan(x, "multiplies", 1, exclusive=True).tolist() == [1, 1, 2, 6, 24]
+    assert oracle.scan(x, "maximum", -100, exclusive=True).tolist() == [-100, 1, 2, 3, 4]
+
+
+@pytest.mark.parametrize("n", SIZES)
+def test_reduce_reference_test_vectors(golden, n):
+    x = golden[f"reduce_in_{n}"]
+    e = golden[f"reduce_expect_{n}"]
+    assert oracle.reduce(x, "plus", 0, 7) == e[0]
+    if n:
+        assert oracle.reduce(x, "maximum", 0, 0) == e[1]
+        assert oracle.reduce(x, "minimum", 0xFFFFFFFF, 0xFFFFFFFF) == e[2]
+
+
+def test_reduce_capi_vector(golden):
+    x = golden["capi_in"]
+    s, m = golden["capi_expect_sum_max"]
+    assert oracle.reduce(x, "plus", 0, 0) == np.int32(s)
+    assert oracle.reduce(x, "maximum", np.iinfo(np.int32).min,
+                         np.iinfo(np.int32).min) == m
+
+
+def test_reduce_empty_returns_init_and_float_minmax():
+    assert oracle.reduce(np.zeros(0, np.uint32), "plus", 0, 42) == 42
+    f = (np.arange(10000) % 17 - 8).astype(np.float32)   # main.cpp:200-204
+    assert oracle.reduce(f, "maximum", -np.inf, -np.inf) == 8.0
+    assert oracle.reduce(f, "minimum", np.inf, np.inf) == -8.0
+    with pytest.raises(ValueError):
+        oracle.reduce(f, "bit_and", 0, 0)
+
+
+# ---- histogram ---------------------------------------------------------------
+
+@pytest.mark.parametrize("n", SIZES)
+def test_histogram_reference_test_vectors(golden, n):
+    x = golden[f"hist_in_{n}"]
+    np.testing.assert_array_equal(oracle.histogram_even(x, 10, 0.0, 1000.0),
+                                  golden[f"hist_expect_{n}"])
+
+
+def test_histogram_brute_force_float_bins():
+    rng = np.random.default_rng(9)
+    x = (rng.standard_normal(50000) * 3).astype(np.float32)
+    lo, up, bins = np.float32(-4.0), np.float32(5.5), 37
+    scale = np.float32(bins) / (up - lo)
+    expect = np.zeros(bins, np.uint32)
+    for v in x[(x >= lo) & (x < up)]:
+        b = int(np.float32(v - lo) * scale)
+        expect[min(b, bins - 1)] += 1
+    np.testing.assert_array_equal(oracle.histogram_even(x, bins, -4.0, 5.5), expect)
+    with pytest.raises(ValueError):
+        oracle.histogram_even(x, 4, 1.0, 1.0)
