@@ -1,0 +1,174 @@
+#!/usr/bin/env python3
+"""Benchmark of the hot path on one MI355X: stable key-value radix sort of
+2^28 uint32 keys (uniform over 32 bits) with uint32 values 0..n-1, through
+the C ABI (clc_sort).
+
+    python bench.py [--steps K] [--warmup W] [--log2n 28] [--no-cpu]
+
+A step is one clc_sort call on inputs already resident in HBM. Before each
+step the pristine keys/values are restored by a device-to-device copy that
+lies outside the step's hipEvent pair, so `value` counts only the sort.
+Prints one JSON line (metric, value in GKeys/s, roofline of the dominant
+kernel, CPU baseline).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "opencl-sdk_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+METRIC = "GKeys/s + % HBM roofline, 2^28 u32 pairs; learned-bucket vs plain radix"
+
+
+def cpu_baseline(log2n: int, reps: int) -> dict:
+    """The oracle's restatement of the reference sort (4-bit LSD, 8 passes,
+    per-thread histograms) on the host cores, on a bounded sample of the
+    same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle
+    n = 1 << log2n
+    rng = np.random.default_rng(12345)
+    keys = rng.integers(0, 1 << 32, size=n, dtype=np.uint32)
+    vals = np.arange(n, dtype=np.uint32)
+    threads = max(1, min(os.cpu_count() or 1, 16))
+    times = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        oracle.radix_sort(keys, vals, threads=threads)
+        times.append(time.perf_counter() - t)
+    best = min(times)
+    return {"value": round(n / best / 1e9, 4), "unit": "GKeys/s", "cores": threads,
+            "kind": "port",
+            "sample": f"2^{log2n} uniform u32 keys + u32 index values, best of {reps} "
+                      f"runs of the oracle's 4-bit LSD restatement "
+                      f"({threads} threads, {best:.2f} s each)"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--log2n", type=int, default=28)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-log2n", type=int, default=26)
+    ap.add_argument("--cpu-reps", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this benchmark runs on exactly one GPU (asked for {args.gpus}); "
+              "the sort has no multi-GPU path in this build", file=sys.stderr)
+        return 2
+
+    import torch
+    import clc_hip
+
+    n = 1 << args.log2n
+    torch.cuda.set_device(0)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(42)
+    keys0 = torch.randint(-(1 << 31), (1 << 31) - 1, (n,), dtype=torch.int32,
+                          device="cuda", generator=g)
+    vals0 = torch.arange(n, dtype=torch.int32, device="cuda")
+    keys = torch.empty_like(keys0)
+    vals = torch.empty_like(vals0)
+    ctx = clc_hip.Context()
+
+    def step():
+        ctx.sort(keys, vals, key_type=clc_hip.U32)
+
+    for _ in range(args.warmup):
+        keys.copy_(keys0)
+        vals.copy_(vals0)
+        step()
+    torch.cuda.synchronize()
+
+    ctx.reset_timings()
+    ctx.set_profiling(True)
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        keys.copy_(keys0)
+        vals.copy_(vals0)
+        starts[i].record()
+        step()
+        ends[i].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ctx.set_profiling(False)
+    step_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    ms = statistics.fmean(step_ms)
+    timings = ctx.kernel_timings()
+
+    # Output check on the last step: ascending keys, values a permutation
+    # consistent with the keys, ties in index order (stable).
+    k = keys.to(torch.int64) & 0xFFFFFFFF
+    ok = bool((k[1:] >= k[:-1]).all())
+    ok = ok and bool((keys0[vals.long()] == keys).all())
+    ties = k[1:] == k[:-1]
+    ok = ok and bool((vals[1:][ties] > vals[:-1][ties]).all())
+
+    one = timings.get("radix_onesweep_pairs")
+    roofline = None
+    if one and one["launches"]:
+        avg_ms = one["total_ms"] / one["launches"]
+        bytes_per_launch = n * 2 * (4 + 4)  # read + write of key and value
+        achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
+        traffic = None
+        tpath = os.path.join(ROOT, "profiles", "onesweep_pairs_2p28_traffic.json")
+        if os.path.exists(tpath):
+            with open(tpath) as f:
+                traffic = json.load(f).get("hbm_bytes_per_launch")
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": traffic, "kernel": "radix_onesweep_pairs",
+                    "avg_launch_ms": round(avg_ms, 4),
+                    "algorithmic_bytes_per_launch": bytes_per_launch}
+    sort_bytes = n * (4 + 4 * 2 * (4 + 4))  # k + P*2*(k+v), P = 4
+    value = n / (ms / 1e3) / 1e9
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "GKeys/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (torch.randint seeded, uniform 32-bit keys; values 0..n-1)",
+        "config": {"workload": f"stable key-value radix sort, 2^{args.log2n} uint32 keys "
+                               "uniform over 32 bits + uint32 index values, ascending",
+                   "n": n, "key_bytes": 4, "value_bytes": 4, "passes": 4,
+                   "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": None,
+        "extra": {"sort_roofline_frac": round(sort_bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                  "step_ms_min": round(min(step_ms), 4),
+                  "step_ms_median": round(statistics.median(step_ms), 4),
+                  "step_ms_max": round(max(step_ms), 4),
+                  "wall_s_timed_loop_incl_restore_copies": round(wall, 3),
+                  "output_check": ok,
+                  "kernels": {k2: {"launches": v2["launches"],
+                                   "avg_ms": round(v2["total_ms"] / max(v2["launches"], 1), 4)}
+                              for k2, v2 in timings.items()}},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(args.cpu_log2n, args.cpu_reps)
+    print(json.dumps(line), flush=True)
+    return 0 if ok else 1
+
+
+if __name__ == "__main__":
+    sys.exit(main())

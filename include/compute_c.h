@@ -1,0 +1,206 @@
+/*
+ * C interface of the MI355X (gfx950) sort / scan / reduce / histogram
+ * library: a drop-in for the radix sort, scan, reduce and histogram entry
+ * points of the OpenCL Compute library (reference
+ * lib/include/CL/Compute/compute_c.h).
+ *
+ * Differences from the reference boundary, all deliberate:
+ *   - clc_buffer handles are replaced by HIP device pointers (void*), and
+ *     the command queue by a hipStream_t bound to the context at creation
+ *     (clc_context_create_from_stream). Bounds cannot be checked on raw
+ *     pointers: the caller guarantees that n elements fit.
+ *   - clc_last_cl_error() returns the hipError_t of the last
+ *     CLC_ERROR_OPENCL, which this library reuses for HIP runtime errors.
+ *   - 64-bit sort keys (I64/U64/F64) are accepted (the reference returns
+ *     CLC_ERROR_INVALID_ARGUMENT for them, CApi.cpp:522-524).
+ *   - Operators are the reference's canonical ops:: expressions (and the
+ *     projections "a", "b", plus "max(a, b)" / "min(a, b)"); any other
+ *     expression returns CLC_ERROR_UNSUPPORTED instead of being compiled.
+ *   - clc_histogram_even, clc_segmented_sort and clc_learned_sort have no C
+ *     entry point in the reference.
+ *
+ * Every function that can fail returns a clc_status; on failure
+ * clc_last_error() describes it until the next call on the same thread. No
+ * C++ exception crosses this boundary. Contexts are not thread-safe: use one
+ * per thread. sort, scan, histogram and the segmented/learned sorts are
+ * asynchronous on the context's stream; reduce blocks until its result is on
+ * the host, as in the reference (Algorithm.cpp:440-441).
+ */
+#ifndef CLC_HIP_COMPUTE_C_H
+#define CLC_HIP_COMPUTE_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct clc_context_s* clc_context;
+
+/* Status codes, numerically identical to the reference
+ * (compute_c.h:30-41). */
+typedef enum clc_status
+{
+    CLC_SUCCESS = 0,
+    CLC_ERROR_OPENCL = 1, /* HIP runtime error: clc_last_cl_error() has it */
+    CLC_ERROR_INVALID_ARGUMENT = 2,
+    CLC_ERROR_BUILD = 3,
+    CLC_ERROR_NO_DEVICE = 4,
+    CLC_ERROR_OUT_OF_MEMORY = 5,
+    CLC_ERROR_UNSUPPORTED = 6,
+    CLC_ERROR_INTERNAL = 7,
+    CLC_ERROR_IO = 8
+} clc_status;
+
+/* Element types, numerically identical to the reference (compute_c.h:44-56). */
+typedef enum clc_type
+{
+    CLC_TYPE_I8 = 0,
+    CLC_TYPE_U8 = 1,
+    CLC_TYPE_I16 = 2,
+    CLC_TYPE_U16 = 3,
+    CLC_TYPE_I32 = 4,
+    CLC_TYPE_U32 = 5,
+    CLC_TYPE_I64 = 6,
+    CLC_TYPE_U64 = 7,
+    CLC_TYPE_F32 = 8,
+    CLC_TYPE_F64 = 9
+} clc_type;
+
+/* Device index selecting the first GPU (compute_c.h:60). */
+#define CLC_DEFAULT_DEVICE ((size_t)-1)
+
+/* ---- errors (replaces compute_c.h:64-66) ------------------------------- */
+
+const char* clc_last_error(void);
+/* hipError_t of the last CLC_ERROR_OPENCL / CLC_ERROR_OUT_OF_MEMORY raised by
+ * the HIP runtime, else 0. */
+int clc_last_cl_error(void);
+
+/* ---- devices and contexts (replaces compute_c.h:70-99) ------------------ */
+
+typedef struct clc_device_info
+{
+    char name[256];
+    char platform[256]; /* "HIP" */
+    int is_gpu;
+    int has_fp64;
+    unsigned long long global_memory;
+    size_t max_work_group_size;
+} clc_device_info;
+
+/* HIP devices. Returns CLC_SUCCESS with *count = 0 when there is none
+ * (reference clc_device_count, compute_c.h:82, CApi.cpp:228-233). */
+clc_status clc_device_count(size_t* count);
+/* (reference clc_device_get_info, compute_c.h:83-84, CApi.cpp:235-244) */
+clc_status clc_device_get_info(size_t index, clc_device_info* info);
+
+/* Creates a context on a device: a new in-order hipStream_t, a pool memory
+ * resource for temporary storage and the launch-configuration cache.
+ * (reference clc_context_create, compute_c.h:89, CApi.cpp:246-249) */
+clc_status clc_context_create(size_t index, clc_context* out);
+/* Same; cache_dir (an existing directory, or NULL/"" for the environment
+ * variable CL_COMPUTE_CACHE_DIR) persists the autotuner's launch
+ * configurations instead of compiled programs.
+ * (reference clc_context_create_with_cache, compute_c.h:92-94,
+ * CApi.cpp:251-283) */
+clc_status clc_context_create_with_cache(size_t index, const char* cache_dir,
+                                         clc_context* out);
+/* Binds a caller-owned hipStream_t (NULL = the null stream) on the current
+ * device. The queue-to-stream swap of compute_context(cl::CommandQueue)
+ * (reference Context.hpp:33-36, MemoryResource.cpp:169-185). */
+clc_status clc_context_create_from_stream(void* hip_stream, clc_context* out);
+/* (reference compute_c.h:95) */
+void clc_context_release(clc_context ctx);
+/* (reference compute_c.h:96-97) */
+clc_status clc_context_get_info(clc_context ctx, clc_device_info* info);
+/* hipStreamSynchronize on the context's stream (reference compute_c.h:99). */
+clc_status clc_finish(clc_context ctx);
+/* The context's hipStream_t. */
+clc_status clc_context_get_stream(clc_context ctx, void** hip_stream);
+
+/* Allocation statistics of the context's memory resource (the reference's
+ * statistics_adaptor counters, MemoryResource.hpp:123-130, plus the pool's
+ * cached bytes, MemoryResource.hpp:100-101). */
+typedef struct clc_memory_stats
+{
+    size_t current_bytes;
+    size_t peak_bytes;
+    size_t total_bytes;
+    size_t current_allocations;
+    size_t total_allocations;
+    size_t cached_bytes;
+} clc_memory_stats;
+clc_status clc_context_memory_stats(clc_context ctx, clc_memory_stats* stats);
+/* Returns every cached block to the device (pool_resource::release,
+ * MemoryResource.hpp:97-98). Synchronizes the context's stream first. */
+clc_status clc_context_release_cache(clc_context ctx);
+
+/* ---- kernel timing ------------------------------------------------------ */
+/* Per-kernel hipEvent timing on the context's stream, off by default (the
+ * reference's samples time kernels through OpenCL event profiling,
+ * lib/include/CL/Utils/Event.hpp:14-19). */
+typedef struct clc_kernel_timing
+{
+    char name[64];
+    size_t launches;
+    double total_ms;
+    double min_ms;
+    double max_ms;
+} clc_kernel_timing;
+clc_status clc_context_set_profiling(clc_context ctx, int enable);
+/* Waits for the timed launches and copies up to capacity entries; *count
+ * receives the number of distinct kernels timed so far. */
+clc_status clc_context_kernel_timings(clc_context ctx, clc_kernel_timing* out,
+                                      size_t capacity, size_t* count);
+clc_status clc_context_reset_timings(clc_context ctx);
+
+/* ---- algorithms (replaces compute_c.h:150-171) -------------------------- */
+/* in/out/keys/values/histogram are device pointers; init/result point to
+ * one host element of the given type. */
+
+/* result = op(init, in[0] op ... op in[n-1]); n == 0 gives init. op must be
+ * associative and commutative. Blocks until the result is on the host.
+ * (reference clc_reduce, compute_c.h:156-159, CApi.cpp:486-499) */
+clc_status clc_reduce(clc_context ctx, const void* in, size_t n, clc_type type,
+                      const char* op, const char* identity, const void* init,
+                      void* result);
+
+/* Inclusive (exclusive = 0) or exclusive prefix scan; out may alias in; the
+ * exclusive scan starts from identity. Associative operators, commutative or
+ * not. (reference clc_scan, compute_c.h:160-163, CApi.cpp:501-515) */
+clc_status clc_scan(clc_context ctx, const void* in, void* out, size_t n,
+                    clc_type type, const char* op, const char* identity,
+                    int exclusive);
+
+/* Stable in-place radix sort of n keys (I32, U32, F32, and as an extension
+ * I64, U64, F64) by the reference's order (Algorithm.cpp:459-469: floats by
+ * their sign-flipped bits, so -0.0 sorts before +0.0); values may be NULL or
+ * hold n 32-bit elements permuted alongside. descending != 0 sorts by the
+ * complemented key, keeping equal keys in input order. n <= 2^31-1.
+ * (reference clc_sort, compute_c.h:166-168, CApi.cpp:517-531) */
+clc_status clc_sort(clc_context ctx, void* keys, clc_type key_type,
+                    void* values, size_t n, int descending);
+
+/* Zeroes num_bins uint32 counters, then counts the samples with
+ * lower <= x < upper into bin (uint)((x - lower) * scale), scale =
+ * bins / (upper - lower), computed in float (double for F64 samples) and
+ * clamped to num_bins - 1. lower >= upper is CLC_ERROR_INVALID_ARGUMENT.
+ * (reference detail::histogram_even, Algorithm.hpp:112-116,
+ * Algorithm.cpp:569-615; no C entry point there) */
+clc_status clc_histogram_even(clc_context ctx, const void* in, size_t n,
+                              clc_type type, uint32_t* histogram,
+                              size_t num_bins, double lower, double upper);
+
+/* Whether clc_scan / clc_reduce support op/identity for type:
+ * CLC_SUCCESS, CLC_ERROR_UNSUPPORTED (expression not recognised) or
+ * CLC_ERROR_INVALID_ARGUMENT. Needs no device. */
+clc_status clc_operator_supported(clc_type type, const char* op,
+                                  const char* identity);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

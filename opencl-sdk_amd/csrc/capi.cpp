@@ -1,0 +1,363 @@
+// The C ABI (include/compute_c.h). Error behaviour follows the reference's
+// lib/src/Compute/CApi.cpp: every call runs inside guarded() (:60-115),
+// which maps each exception type to a clc_status and keeps a thread-local
+// message (:49-57) cleared on success; device enumeration and context
+// creation are serialized by one mutex (:117-124).
+#include "../../include/compute_c.h"
+
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <string>
+
+#include "context.hpp"
+#include "ops.hpp"
+
+struct clc_context_s
+{
+    std::unique_ptr<clc::context> ctx;
+};
+
+namespace {
+
+thread_local std::string last_error;
+thread_local int last_hip_error = 0;
+
+clc_status fail(clc_status status, const std::string& message, int code = 0)
+{
+    last_error = message;
+    last_hip_error = code;
+    return status;
+}
+
+template <typename F> clc_status guarded(F&& body)
+{
+    try
+    {
+        body();
+        last_error.clear();
+        last_hip_error = 0;
+        return CLC_SUCCESS;
+    } catch (const clc::no_device_error& e)
+    {
+        return fail(CLC_ERROR_NO_DEVICE, e.what());
+    } catch (const clc::unsupported_error& e)
+    {
+        return fail(CLC_ERROR_UNSUPPORTED, e.what());
+    } catch (const clc::out_of_memory& e)
+    {
+        return fail(CLC_ERROR_OUT_OF_MEMORY, e.what(),
+                    static_cast<int>(hipErrorOutOfMemory));
+    } catch (const clc::hip_error& e)
+    {
+        return fail(CLC_ERROR_OPENCL, e.what(), static_cast<int>(e.code));
+    } catch (const std::bad_alloc&)
+    {
+        return fail(CLC_ERROR_OUT_OF_MEMORY, "host allocation failed");
+    } catch (const std::invalid_argument& e)
+    {
+        return fail(CLC_ERROR_INVALID_ARGUMENT, e.what());
+    } catch (const std::out_of_range& e)
+    {
+        return fail(CLC_ERROR_INVALID_ARGUMENT, e.what());
+    } catch (const std::length_error& e)
+    {
+        return fail(CLC_ERROR_INVALID_ARGUMENT, e.what());
+    } catch (const std::overflow_error& e)
+    {
+        return fail(CLC_ERROR_OUT_OF_MEMORY, e.what());
+    } catch (const std::exception& e)
+    {
+        return fail(CLC_ERROR_INTERNAL, e.what());
+    } catch (...)
+    {
+        return fail(CLC_ERROR_INTERNAL, "unknown error");
+    }
+}
+
+std::mutex& init_mutex()
+{
+    static std::mutex m;
+    return m;
+}
+
+void require(bool condition, const char* message)
+{
+    if (!condition) throw std::invalid_argument(message);
+}
+
+std::size_t device_count_locked()
+{
+    int count = 0;
+    const hipError_t e = hipGetDeviceCount(&count);
+    if (e == hipErrorNoDevice || e == hipErrorInsufficientDriver
+        || e == hipErrorInvalidDevice)
+    {
+        (void)hipGetLastError();
+        return 0;
+    }
+    clc::hip_check(e, "hipGetDeviceCount");
+    return count < 0 ? 0 : static_cast<std::size_t>(count);
+}
+
+void copy_text(char (&dst)[256], const std::string& src)
+{
+    const std::size_t n = src.size() < 255 ? src.size() : 255;
+    std::memcpy(dst, src.data(), n);
+    dst[n] = '\0';
+}
+
+void fill_info(int device, clc_device_info* info)
+{
+    hipDeviceProp_t prop;
+    CLC_HIP(hipGetDeviceProperties(&prop, device));
+    std::memset(info, 0, sizeof *info);
+    copy_text(info->name, std::string(prop.name) + " (" + prop.gcnArchName + ")");
+    copy_text(info->platform, "HIP");
+    info->is_gpu = 1;
+    info->has_fp64 = 1;
+    info->global_memory = prop.totalGlobalMem;
+    info->max_work_group_size = static_cast<std::size_t>(prop.maxThreadsPerBlock);
+}
+
+clc::context& get(clc_context ctx)
+{
+    require(ctx != nullptr && ctx->ctx != nullptr, "clc: null context");
+    return *ctx->ctx;
+}
+
+std::string env_cache_dir(const char* dir)
+{
+    if (dir != nullptr && *dir != '\0') return dir;
+    const char* env = std::getenv("CL_COMPUTE_CACHE_DIR");
+    return env != nullptr ? env : "";
+}
+
+} // namespace
+
+extern "C" {
+
+const char* clc_last_error(void) { return last_error.c_str(); }
+
+int clc_last_cl_error(void) { return last_hip_error; }
+
+clc_status clc_device_count(size_t* count)
+{
+    if (count == nullptr)
+        return fail(CLC_ERROR_INVALID_ARGUMENT, "clc: null count");
+    return guarded([&] {
+        std::lock_guard<std::mutex> lock(init_mutex());
+        *count = device_count_locked();
+    });
+}
+
+clc_status clc_device_get_info(size_t index, clc_device_info* info)
+{
+    if (info == nullptr)
+        return fail(CLC_ERROR_INVALID_ARGUMENT, "clc: null info");
+    return guarded([&] {
+        std::lock_guard<std::mutex> lock(init_mutex());
+        require(index < device_count_locked(), "clc: device index out of range");
+        fill_info(static_cast<int>(index), info);
+    });
+}
+
+clc_status clc_context_create(size_t index, clc_context* out)
+{
+    return clc_context_create_with_cache(index, nullptr, out);
+}
+
+clc_status clc_context_create_with_cache(size_t index, const char* cache_dir,
+                                         clc_context* out)
+{
+    if (out == nullptr)
+        return fail(CLC_ERROR_INVALID_ARGUMENT, "clc: null output");
+    *out = nullptr;
+    return guarded([&] {
+        std::lock_guard<std::mutex> lock(init_mutex());
+        const std::size_t count = device_count_locked();
+        if (count == 0) throw clc::no_device_error("clc: no HIP device available");
+        if (index == CLC_DEFAULT_DEVICE) index = 0; // every HIP device is a GPU
+        if (index >= count)
+            throw clc::no_device_error("clc: device index out of range");
+        std::unique_ptr<clc_context_s> c(new clc_context_s);
+        c->ctx.reset(new clc::context(static_cast<int>(index), nullptr, true,
+                                      env_cache_dir(cache_dir)));
+        *out = c.release();
+    });
+}
+
+clc_status clc_context_create_from_stream(void* hip_stream, clc_context* out)
+{
+    if (out == nullptr)
+        return fail(CLC_ERROR_INVALID_ARGUMENT, "clc: null output");
+    *out = nullptr;
+    return guarded([&] {
+        std::lock_guard<std::mutex> lock(init_mutex());
+        if (device_count_locked() == 0)
+            throw clc::no_device_error("clc: no HIP device available");
+        int device = 0;
+        CLC_HIP(hipGetDevice(&device));
+        std::unique_ptr<clc_context_s> c(new clc_context_s);
+        c->ctx.reset(new clc::context(device, static_cast<hipStream_t>(hip_stream),
+                                      false, env_cache_dir(nullptr)));
+        *out = c.release();
+    });
+}
+
+void clc_context_release(clc_context ctx) { delete ctx; }
+
+clc_status clc_context_get_info(clc_context ctx, clc_device_info* info)
+{
+    if (ctx == nullptr || info == nullptr)
+        return fail(CLC_ERROR_INVALID_ARGUMENT, "clc: null argument");
+    return guarded([&] { fill_info(get(ctx).device(), info); });
+}
+
+clc_status clc_finish(clc_context ctx)
+{
+    if (ctx == nullptr)
+        return fail(CLC_ERROR_INVALID_ARGUMENT, "clc: null context");
+    return guarded([&] { CLC_HIP(hipStreamSynchronize(get(ctx).stream())); });
+}
+
+clc_status clc_context_get_stream(clc_context ctx, void** hip_stream)
+{
+    if (ctx == nullptr || hip_stream == nullptr)
+        return fail(CLC_ERROR_INVALID_ARGUMENT, "clc: null argument");
+    return guarded([&] { *hip_stream = get(ctx).stream(); });
+}
+
+clc_status clc_context_memory_stats(clc_context ctx, clc_memory_stats* stats)
+{
+    if (ctx == nullptr || stats == nullptr)
+        return fail(CLC_ERROR_INVALID_ARGUMENT, "clc: null argument");
+    return guarded([&] {
+        clc::context& c = get(ctx);
+        const auto s = c.stats().statistics();
+        stats->current_bytes = s.current_bytes;
+        stats->peak_bytes = s.peak_bytes;
+        stats->total_bytes = s.total_bytes;
+        stats->current_allocations = s.current_allocations;
+        stats->total_allocations = s.total_allocations;
+        stats->cached_bytes = c.pool().cached_bytes();
+    });
+}
+
+clc_status clc_context_release_cache(clc_context ctx)
+{
+    if (ctx == nullptr)
+        return fail(CLC_ERROR_INVALID_ARGUMENT, "clc: null context");
+    return guarded([&] {
+        clc::context& c = get(ctx);
+        CLC_HIP(hipStreamSynchronize(c.stream()));
+        c.pool().release();
+    });
+}
+
+clc_status clc_context_set_profiling(clc_context ctx, int enable)
+{
+    if (ctx == nullptr)
+        return fail(CLC_ERROR_INVALID_ARGUMENT, "clc: null context");
+    return guarded([&] { get(ctx).profiler().enable(enable != 0); });
+}
+
+clc_status clc_context_kernel_timings(clc_context ctx, clc_kernel_timing* out,
+                                      size_t capacity, size_t* count)
+{
+    if (ctx == nullptr || count == nullptr || (capacity != 0 && out == nullptr))
+        return fail(CLC_ERROR_INVALID_ARGUMENT, "clc: null argument");
+    return guarded([&] {
+        const auto& totals = get(ctx).profiler().collect();
+        std::size_t i = 0;
+        for (const auto& kv : totals)
+        {
+            if (i < capacity)
+            {
+                clc_kernel_timing& t = out[i];
+                std::memset(&t, 0, sizeof t);
+                const std::size_t len = kv.first.size() < 63 ? kv.first.size() : 63;
+                std::memcpy(t.name, kv.first.data(), len);
+                t.launches = kv.second.launches;
+                t.total_ms = kv.second.total_ms;
+                t.min_ms = kv.second.min_ms;
+                t.max_ms = kv.second.max_ms;
+            }
+            ++i;
+        }
+        *count = totals.size();
+    });
+}
+
+clc_status clc_context_reset_timings(clc_context ctx)
+{
+    if (ctx == nullptr)
+        return fail(CLC_ERROR_INVALID_ARGUMENT, "clc: null context");
+    return guarded([&] { get(ctx).profiler().reset(); });
+}
+
+clc_status clc_reduce(clc_context ctx, const void* in, size_t n, clc_type type,
+                      const char* op, const char* identity, const void* init,
+                      void* result)
+{
+    return guarded([&] {
+        require(ctx != nullptr && op != nullptr && identity != nullptr
+                    && init != nullptr && result != nullptr,
+                "clc: null argument");
+        const clc::op_code code = clc::parse_op(op, type);
+        unsigned char id[8];
+        clc::parse_identity(identity, type, id);
+        clc::detail::reduce(get(ctx), in, n, type, code, id, init, result);
+    });
+}
+
+clc_status clc_scan(clc_context ctx, const void* in, void* out, size_t n,
+                    clc_type type, const char* op, const char* identity,
+                    int exclusive)
+{
+    return guarded([&] {
+        require(ctx != nullptr && op != nullptr && identity != nullptr,
+                "clc: null argument");
+        const clc::op_code code = clc::parse_op(op, type);
+        unsigned char id[8];
+        clc::parse_identity(identity, type, id);
+        clc::detail::scan(get(ctx), in, out, n, type, code, id, exclusive != 0);
+    });
+}
+
+clc_status clc_sort(clc_context ctx, void* keys, clc_type key_type,
+                    void* values, size_t n, int descending)
+{
+    return guarded([&] {
+        require(ctx != nullptr, "clc: null context");
+        clc::detail::radix_sort(get(ctx), keys, key_type, values, n,
+                                descending != 0);
+    });
+}
+
+clc_status clc_histogram_even(clc_context ctx, const void* in, size_t n,
+                              clc_type type, uint32_t* histogram,
+                              size_t num_bins, double lower, double upper)
+{
+    return guarded([&] {
+        require(ctx != nullptr, "clc: null context");
+        clc::detail::histogram_even(get(ctx), in, n, type, histogram, num_bins,
+                                    lower, upper);
+    });
+}
+
+clc_status clc_operator_supported(clc_type type, const char* op,
+                                  const char* identity)
+{
+    return guarded([&] {
+        require(op != nullptr && identity != nullptr, "clc: null argument");
+        clc::type_size(type);
+        clc::parse_op(op, type);
+        unsigned char id[8];
+        clc::parse_identity(identity, type, id);
+    });
+}
+
+} // extern "C"

@@ -1,0 +1,147 @@
+#include "context.hpp"
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+
+namespace clc {
+
+void hip_check(hipError_t e, const char* what)
+{
+    if (e == hipSuccess) return;
+    (void)hipGetLastError(); // clear the sticky per-thread error state
+    const std::string msg =
+        std::string(what) + " failed: " + hipGetErrorString(e) + " ("
+        + std::to_string(static_cast<int>(e)) + ")";
+    if (e == hipErrorOutOfMemory) throw out_of_memory(msg);
+    if (e == hipErrorNoDevice || e == hipErrorInvalidDevice)
+        throw no_device_error(msg);
+    throw hip_error(e, msg);
+}
+
+void* device_resource::do_allocate(std::size_t bytes)
+{
+    int prev = 0;
+    CLC_HIP(hipGetDevice(&prev));
+    if (prev != device_) CLC_HIP(hipSetDevice(device_));
+    void* p = nullptr;
+    const hipError_t e = hipMalloc(&p, bytes == 0 ? 1 : bytes);
+    if (prev != device_) (void)hipSetDevice(prev);
+    hip_check(e, "hipMalloc");
+    return p;
+}
+
+void device_resource::do_deallocate(void* p, std::size_t) noexcept
+{
+    (void)hipFree(p);
+}
+
+context::context(int device, hipStream_t stream, bool owns_stream,
+                 std::string cache_dir)
+    : device_(device), stream_(stream), owns_stream_(owns_stream),
+      cache_dir_(std::move(cache_dir))
+{
+    CLC_HIP(hipSetDevice(device_));
+    hipDeviceProp_t prop;
+    CLC_HIP(hipGetDeviceProperties(&prop, device_));
+    facts_.name = prop.name;
+    facts_.arch = prop.gcnArchName;
+    facts_.cu_count = prop.multiProcessorCount;
+    facts_.lds_per_block = prop.sharedMemPerBlock;
+    facts_.global_memory = prop.totalGlobalMem;
+    facts_.max_threads_per_block = prop.maxThreadsPerBlock;
+    facts_.xcd_count =
+        facts_.arch.compare(0, 5, "gfx95") == 0
+                || facts_.arch.compare(0, 5, "gfx94") == 0
+            ? 8
+            : 1;
+    if (owns_stream_) CLC_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    stats_ = std::make_shared<statistics_adaptor>(
+        std::make_shared<device_resource>(device_));
+    pool_ = std::make_shared<pool_resource>(stats_);
+    resource_ = pool_;
+    CLC_HIP(hipHostMalloc(&pinned_, 64, hipHostMallocDefault));
+}
+
+context::~context()
+{
+    (void)hipSetDevice(device_);
+    (void)hipStreamSynchronize(stream_);
+    if (pinned_) (void)hipHostFree(pinned_);
+    resource_.reset();
+    pool_.reset(); // returns cached blocks
+    stats_.reset();
+    if (owns_stream_) (void)hipStreamDestroy(stream_);
+}
+
+profiler::~profiler()
+{
+    for (auto& p : pending_)
+    {
+        (void)hipEventDestroy(p.begin);
+        (void)hipEventDestroy(p.end);
+    }
+    for (hipEvent_t e : free_) (void)hipEventDestroy(e);
+}
+
+hipEvent_t profiler::get_event()
+{
+    if (!free_.empty())
+    {
+        hipEvent_t e = free_.back();
+        free_.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    CLC_HIP(hipEventCreate(&e));
+    return e;
+}
+
+int profiler::start(const char* name, hipStream_t stream)
+{
+    if (!enabled_) return -1;
+    pending p{ name, get_event(), get_event() };
+    CLC_HIP(hipEventRecord(p.begin, stream));
+    pending_.push_back(p);
+    return static_cast<int>(pending_.size() - 1);
+}
+
+void profiler::stop(int token, hipStream_t stream)
+{
+    if (token < 0 || token >= static_cast<int>(pending_.size())) return;
+    (void)hipEventRecord(pending_[token].end, stream);
+}
+
+const std::map<std::string, profiler::stats>& profiler::collect()
+{
+    for (auto& p : pending_)
+    {
+        CLC_HIP(hipEventSynchronize(p.end));
+        float ms = 0.0f;
+        CLC_HIP(hipEventElapsedTime(&ms, p.begin, p.end));
+        stats& s = totals_[p.name];
+        s.min_ms = s.launches == 0 ? ms : std::min(s.min_ms, double(ms));
+        s.max_ms = s.launches == 0 ? ms : std::max(s.max_ms, double(ms));
+        s.total_ms += ms;
+        ++s.launches;
+        free_.push_back(p.begin);
+        free_.push_back(p.end);
+    }
+    pending_.clear();
+    return totals_;
+}
+
+void profiler::reset()
+{
+    collect();
+    totals_.clear();
+}
+
+void context::activate() const
+{
+    int cur = -1;
+    CLC_HIP(hipGetDevice(&cur));
+    if (cur != device_) CLC_HIP(hipSetDevice(device_));
+}
+
+} // namespace clc

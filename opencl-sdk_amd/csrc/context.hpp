@@ -1,0 +1,184 @@
+// Execution context and launch layer: one HIP device, one in-order stream,
+// a pool memory resource and the device facts the launch configurations
+// depend on. Plays the role of the reference's compute_context
+// (lib/include/CL/Compute/Context.hpp:25-59, MemoryResource.cpp:169-197)
+// with the command queue replaced by a hipStream_t.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "pool.hpp"
+
+namespace clc {
+
+//! A failed HIP runtime call (CLC_ERROR_OPENCL at the C boundary).
+struct hip_error : std::runtime_error
+{
+    hip_error(hipError_t code, const std::string& what)
+        : std::runtime_error(what), code(code)
+    {}
+    hipError_t code;
+};
+
+struct no_device_error : std::runtime_error
+{
+    using std::runtime_error::runtime_error;
+};
+
+//! Throws hip_error (or out_of_memory) when e is not hipSuccess.
+void hip_check(hipError_t e, const char* what);
+
+#define CLC_HIP(call) ::clc::hip_check((call), #call)
+
+//! Upstream resource calling hipMalloc / hipFree for every block
+//! (buffer_resource, MemoryResource.hpp:52-61, MemoryResource.cpp:12-25).
+class device_resource final : public memory_resource {
+public:
+    explicit device_resource(int device) : device_(device) {}
+
+protected:
+    void* do_allocate(std::size_t bytes) override;
+    void do_deallocate(void* p, std::size_t bytes) noexcept override;
+
+private:
+    int device_;
+};
+
+//! Per-kernel launch timing with hipEvent pairs on the context's stream
+//! (the reference's samples time kernels with event profiling,
+//! lib/include/CL/Utils/Event.hpp:14-19). Off by default; when on, each
+//! timed launch is bracketed by two events, and collect() waits for them.
+class profiler {
+public:
+    struct stats
+    {
+        std::size_t launches = 0;
+        double total_ms = 0.0, min_ms = 0.0, max_ms = 0.0;
+    };
+    ~profiler();
+    bool enabled() const { return enabled_; }
+    void enable(bool on) { enabled_ = on; }
+    //! Records the start event; returns a token for stop(), or -1 when off.
+    int start(const char* name, hipStream_t stream);
+    void stop(int token, hipStream_t stream);
+    //! Waits for outstanding events and folds them into the totals.
+    const std::map<std::string, stats>& collect();
+    void reset();
+
+private:
+    struct pending
+    {
+        std::string name;
+        hipEvent_t begin, end;
+    };
+    hipEvent_t get_event();
+    bool enabled_ = false;
+    std::vector<pending> pending_;
+    std::vector<hipEvent_t> free_;
+    std::map<std::string, stats> totals_;
+};
+
+struct device_facts
+{
+    std::string name;
+    int cu_count = 256;
+    int xcd_count = 8;
+    std::size_t lds_per_block = 65536;
+    std::size_t global_memory = 0;
+    int max_threads_per_block = 1024;
+    std::string arch;
+};
+
+class context {
+public:
+    //! Creates a stream on device (owned) or binds stream (not owned).
+    context(int device, hipStream_t stream, bool owns_stream,
+            std::string cache_dir);
+    ~context();
+    context(const context&) = delete;
+    context& operator=(const context&) = delete;
+
+    int device() const { return device_; }
+    hipStream_t stream() const { return stream_; }
+    const device_facts& facts() const { return facts_; }
+    const std::shared_ptr<memory_resource>& resource() const
+    {
+        return resource_;
+    }
+    pool_resource& pool() { return *pool_; }
+    statistics_adaptor& stats() { return *stats_; }
+    const std::string& cache_dir() const { return cache_dir_; }
+
+    //! Pinned host staging for small blocking reads (reduce results).
+    void* pinned() const { return pinned_; }
+
+    //! Makes this context's device current for the calling thread.
+    void activate() const;
+
+    class profiler& profiler() { return profiler_; }
+
+private:
+    int device_;
+    hipStream_t stream_;
+    bool owns_stream_;
+    std::string cache_dir_;
+    device_facts facts_;
+    std::shared_ptr<statistics_adaptor> stats_;
+    std::shared_ptr<pool_resource> pool_;
+    std::shared_ptr<memory_resource> resource_;
+    void* pinned_ = nullptr;
+    class profiler profiler_;
+};
+
+//! Brackets one kernel launch with profiler events when profiling is on.
+class timed_launch {
+public:
+    timed_launch(context& ctx, const char* name)
+        : ctx_(ctx), token_(ctx.profiler().start(name, ctx.stream()))
+    {}
+    ~timed_launch()
+    {
+        if (token_ >= 0) ctx_.profiler().stop(token_, ctx_.stream());
+    }
+    timed_launch(const timed_launch&) = delete;
+    timed_launch& operator=(const timed_launch&) = delete;
+
+private:
+    context& ctx_;
+    int token_;
+};
+
+//! Kernels index sizes up to 2^31-1 per call, as the reference
+//! (lib/src/Compute/Detail.hpp:18-23).
+inline void check_size(std::size_t n)
+{
+    if (n > 0x7FFFFFFFu)
+        throw std::length_error("clc: at most 2^31-1 elements per call");
+}
+
+namespace detail {
+
+// Entry points behind the C ABI (the reference's detail:: functions,
+// Algorithm.hpp:96-116). Pointers are device pointers.
+void radix_sort(context& ctx, void* keys, int key_type, void* values,
+                std::size_t n, bool descending);
+void scan(context& ctx, const void* in, void* out, std::size_t n, int type,
+          int op, const void* identity, bool exclusive);
+void reduce(context& ctx, const void* in, std::size_t n, int type, int op,
+            const void* identity, const void* init, void* result);
+void histogram_even(context& ctx, const void* in, std::size_t n, int type,
+                    uint32_t* histogram, std::size_t num_bins, double lower,
+                    double upper);
+//! Zeroes bytes of device memory with a kernel (graph-capturable, unlike a
+//! memset node).
+void zero_fill(hipStream_t stream, void* p, std::size_t bytes);
+
+} // namespace detail
+
+} // namespace clc

@@ -1,0 +1,101 @@
+"""C-ABI checks that need no GPU: the library loads, exports every function
+include/compute_c.h declares, and reports errors the way the reference's
+C API does (lib/src/Compute/CApi.cpp:60-115, lib/test/Compute/capi.c)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import clc_hip
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", fn)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names |= set(re.findall(r"\b(clc_[a-z0-9_]+)\s*\(", text))
+    return names
+
+
+def test_every_declared_symbol_is_exported():
+    L = clc_hip.lib()
+    declared = declared_functions()
+    assert len(declared) >= 18
+    missing = [n for n in declared if not hasattr(L, n)]
+    assert not missing, missing
+    # The binding covers every declared function.
+    assert declared == set(clc_hip.SIGNATURES)
+
+
+def test_null_arguments_are_invalid_argument():
+    L = clc_hip.lib()
+    assert L.clc_device_count(None) == clc_hip.ERROR_INVALID_ARGUMENT
+    assert "null" in L.clc_last_error().decode()
+    assert L.clc_context_create(0, None) == clc_hip.ERROR_INVALID_ARGUMENT
+    assert L.clc_finish(None) == clc_hip.ERROR_INVALID_ARGUMENT
+    assert L.clc_sort(None, None, clc_hip.U32, None, 10, 0) == clc_hip.ERROR_INVALID_ARGUMENT
+    assert L.clc_scan(None, None, None, 10, clc_hip.U32, b"(a + b)", b"0", 0) \
+        == clc_hip.ERROR_INVALID_ARGUMENT
+    x = ctypes.c_uint32(0)
+    assert L.clc_reduce(None, None, 10, clc_hip.U32, b"(a + b)", b"0",
+                        ctypes.byref(x), ctypes.byref(x)) == clc_hip.ERROR_INVALID_ARGUMENT
+    assert L.clc_histogram_even(None, None, 10, clc_hip.U32, None, 4, 0.0, 1.0) \
+        == clc_hip.ERROR_INVALID_ARGUMENT
+    L.clc_context_release(None)  # no-op, like the reference
+
+
+def test_last_error_cleared_on_success():
+    L = clc_hip.lib()
+    assert L.clc_operator_supported(clc_hip.U32, b"nonsense", b"0") == clc_hip.ERROR_UNSUPPORTED
+    assert len(L.clc_last_error()) > 0
+    assert L.clc_operator_supported(clc_hip.U32, b"(a + b)", b"0") == clc_hip.SUCCESS
+    assert L.clc_last_error() == b""
+    assert L.clc_last_cl_error() == 0
+
+
+def test_device_enumeration_without_gpu():
+    L = clc_hip.lib()
+    count = ctypes.c_size_t(123)
+    assert L.clc_device_count(ctypes.byref(count)) == clc_hip.SUCCESS
+    info = clc_hip.DeviceInfo()
+    assert L.clc_device_get_info(count.value, ctypes.byref(info)) \
+        == clc_hip.ERROR_INVALID_ARGUMENT
+    if count.value == 0:
+        h = ctypes.c_void_p()
+        assert L.clc_context_create(clc_hip.DEFAULT_DEVICE, ctypes.byref(h)) \
+            == clc_hip.ERROR_NO_DEVICE
+        assert h.value is None
+
+
+@pytest.mark.parametrize("t,op,ident", [
+    (clc_hip.U32, "(a + b)", "0"), (clc_hip.I32, "a+b", "0"),
+    (clc_hip.U32, "(a * b)", "1"), (clc_hip.U32, "(a & b)", "(~0)"),
+    (clc_hip.U64, "(a | b)", "0"), (clc_hip.I8, "(a ^ b)", "0"),
+    (clc_hip.U32, "((b < a) ? b : a)", "UINT_MAX"),
+    (clc_hip.I32, "((a < b) ? b : a)", "INT_MIN"),
+    (clc_hip.I32, "max(a, b)", "INT_MIN"), (clc_hip.F32, "min(a, b)", "INFINITY"),
+    (clc_hip.F32, "((a < b) ? b : a)", "(-INFINITY)"),
+    (clc_hip.F64, "(a + b)", "0.0"), (clc_hip.I32, "b", "0"),
+    (clc_hip.I64, "((b < a) ? b : a)", "LONG_MAX"),
+    (clc_hip.U64, "((a < b) ? b : a)", "0"), (clc_hip.F32, "(a*b)", "1.0f"),
+])
+def test_canonical_operators_supported(t, op, ident):
+    assert clc_hip.lib().clc_operator_supported(t, op.encode(), ident.encode()) \
+        == clc_hip.SUCCESS, clc_hip.lib().clc_last_error()
+
+
+@pytest.mark.parametrize("t,op,ident,status", [
+    (clc_hip.U32, "a - b", "0", clc_hip.ERROR_UNSUPPORTED),
+    (clc_hip.U32, "x +* nonsense", "0", clc_hip.ERROR_UNSUPPORTED),
+    (clc_hip.F32, "(a & b)", "0", clc_hip.ERROR_INVALID_ARGUMENT),
+    (clc_hip.U32, "(a + b)", "zero", clc_hip.ERROR_UNSUPPORTED),
+    (clc_hip.F32, "(a + b)", "1 + 2", clc_hip.ERROR_UNSUPPORTED),
+    (42, "(a + b)", "0", clc_hip.ERROR_INVALID_ARGUMENT),
+])
+def test_unsupported_operators(t, op, ident, status):
+    assert clc_hip.lib().clc_operator_supported(t, op.encode(), ident.encode()) == status
