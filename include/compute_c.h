@@ -193,6 +193,52 @@ clc_status clc_histogram_even(clc_context ctx, const void* in, size_t n,
                               clc_type type, uint32_t* histogram,
                               size_t num_bins, double lower, double upper);
 
+/* ---- segmented sort (no counterpart in the reference) -------------------- */
+/* The reference's equivalent is one clc_sort call per segment
+ * (CApi.cpp:517-531). */
+
+/* Stable sort of every segment [offsets[i], offsets[i+1]) of keys (and the
+ * 32-bit values alongside, if values != NULL), each independently, by the
+ * same key order as clc_sort. offsets is a device array of num_segments + 1
+ * non-decreasing uint64 offsets, offsets[num_segments] <= num_items; each
+ * segment holds fewer than 2^30 keys. Segments that break this contract are
+ * left untouched (never read or written out of bounds). Key types as
+ * clc_sort. Asynchronous; a fixed launch sequence with no host round trip. */
+clc_status clc_segmented_sort(clc_context ctx, void* keys, clc_type key_type,
+                              void* values, size_t num_items,
+                              const uint64_t* offsets, size_t num_segments,
+                              int descending);
+
+/* Times every workgroup configuration for each segment-size bucket
+ * (bucket b: 2^(b-1) < size <= 2^b) on synthetic segments and keeps the
+ * fastest, for this key type with or without values. The result is used by
+ * later calls on ctx and, when the context has a cache directory, written
+ * there (one file per device, key type and values flag; entries from
+ * another device or runtime are ignored and re-tuned). Synchronous. */
+clc_status clc_segmented_sort_tune(clc_context ctx, clc_type key_type,
+                                   int with_values);
+
+/* The size-class table in use: classes[b] for the 32 buckets, values
+ * 0..6 = LDS configurations (64x4, 64x8, 128x8, 256x8, 256x16, 512x16,
+ * 1024x16 threads x keys), 7 = multi-workgroup onesweep. */
+clc_status clc_segmented_sort_get_table(clc_context ctx, clc_type key_type,
+                                        int with_values, int* classes);
+
+/* A segmented sort of fixed buffers captured once into a hipGraph, with its
+ * own scratch; each clc_graph_launch replays the whole sort as one graph
+ * launch on the context's stream. The buffers' contents (keys, values and
+ * offsets) may change between launches; their addresses, num_items and
+ * num_segments may not. The context must outlive the graph. */
+typedef struct clc_graph_s* clc_graph;
+clc_status clc_segmented_sort_graph_create(clc_context ctx, void* keys,
+                                           clc_type key_type, void* values,
+                                           size_t num_items,
+                                           const uint64_t* offsets,
+                                           size_t num_segments, int descending,
+                                           clc_graph* out);
+clc_status clc_graph_launch(clc_graph graph);
+void clc_graph_release(clc_graph graph);
+
 /* Whether clc_scan / clc_reduce support op/identity for type:
  * CLC_SUCCESS, CLC_ERROR_UNSUPPORTED (expression not recognised) or
  * CLC_ERROR_INVALID_ARGUMENT. Needs no device. */

@@ -82,6 +82,13 @@ SIGNATURES = {
     "clc_histogram_even": (_i, [_vp, _vp, _sz, _i, _vp, _sz, ctypes.c_double,
                                 ctypes.c_double]),
     "clc_operator_supported": (_i, [_i, _cp, _cp]),
+    "clc_segmented_sort": (_i, [_vp, _vp, _i, _vp, _sz, _vp, _sz, _i]),
+    "clc_segmented_sort_tune": (_i, [_vp, _i, _i]),
+    "clc_segmented_sort_get_table": (_i, [_vp, _i, _i, ctypes.POINTER(ctypes.c_int)]),
+    "clc_segmented_sort_graph_create": (_i, [_vp, _vp, _i, _vp, _sz, _vp, _sz, _i,
+                                             ctypes.POINTER(_vp)]),
+    "clc_graph_launch": (_i, [_vp]),
+    "clc_graph_release": (None, [_vp]),
 }
 
 _lib = None
@@ -122,6 +129,27 @@ def dtype_code(dtype) -> int:
         if hasattr(torch, name):
             table[getattr(torch, name)] = code
     return table[dtype]
+
+
+class Graph:
+    """A captured segmented sort (clc_graph)."""
+
+    def __init__(self, handle):
+        self.handle = handle
+
+    def launch(self):
+        check(lib().clc_graph_launch(self.handle))
+
+    def close(self):
+        if self.handle:
+            lib().clc_graph_release(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Context:
@@ -180,6 +208,35 @@ class Context:
         check(lib().clc_sort(self.handle, ctypes.c_void_p(keys.data_ptr()), kt,
                              None if values is None else ctypes.c_void_p(values.data_ptr()),
                              n, 1 if descending else 0))
+
+    def segmented_sort(self, keys, values, offsets, descending=False, key_type=None,
+                       num_items=None, num_segments=None):
+        kt = dtype_code(keys.dtype) if key_type is None else key_type
+        n = keys.numel() if num_items is None else num_items
+        ns = offsets.numel() - 1 if num_segments is None else num_segments
+        check(lib().clc_segmented_sort(
+            self.handle, ctypes.c_void_p(keys.data_ptr()), kt,
+            None if values is None else ctypes.c_void_p(values.data_ptr()), n,
+            ctypes.c_void_p(offsets.data_ptr()), ns, 1 if descending else 0))
+
+    def segmented_sort_tune(self, key_type, with_values=True):
+        check(lib().clc_segmented_sort_tune(self.handle, key_type, 1 if with_values else 0))
+
+    def segmented_sort_table(self, key_type, with_values=True):
+        arr = (ctypes.c_int * 32)()
+        check(lib().clc_segmented_sort_get_table(self.handle, key_type,
+                                                 1 if with_values else 0, arr))
+        return list(arr)
+
+    def segmented_sort_graph(self, keys, values, offsets, descending=False, key_type=None):
+        kt = dtype_code(keys.dtype) if key_type is None else key_type
+        h = ctypes.c_void_p()
+        check(lib().clc_segmented_sort_graph_create(
+            self.handle, ctypes.c_void_p(keys.data_ptr()), kt,
+            None if values is None else ctypes.c_void_p(values.data_ptr()), keys.numel(),
+            ctypes.c_void_p(offsets.data_ptr()), offsets.numel() - 1,
+            1 if descending else 0, ctypes.byref(h)))
+        return Graph(h)
 
     def scan(self, inp, out, op="(a + b)", identity="0", exclusive=False,
              type_code=None, n=None):

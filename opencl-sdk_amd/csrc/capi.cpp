@@ -14,10 +14,16 @@
 
 #include "context.hpp"
 #include "ops.hpp"
+#include "segmented_sort.hpp"
 
 struct clc_context_s
 {
     std::unique_ptr<clc::context> ctx;
+};
+
+struct clc_graph_s
+{
+    std::unique_ptr<clc::seg::graph> g;
 };
 
 namespace {
@@ -347,6 +353,67 @@ clc_status clc_histogram_even(clc_context ctx, const void* in, size_t n,
                                     lower, upper);
     });
 }
+
+clc_status clc_segmented_sort(clc_context ctx, void* keys, clc_type key_type,
+                              void* values, size_t num_items,
+                              const uint64_t* offsets, size_t num_segments,
+                              int descending)
+{
+    return guarded([&] {
+        require(ctx != nullptr, "clc: null context");
+        clc::detail::segmented_sort(get(ctx), keys, key_type, values, num_items,
+                                    offsets, num_segments, descending != 0);
+    });
+}
+
+clc_status clc_segmented_sort_tune(clc_context ctx, clc_type key_type,
+                                   int with_values)
+{
+    return guarded([&] {
+        require(ctx != nullptr, "clc: null context");
+        clc::detail::segmented_sort_tune(get(ctx), key_type, with_values != 0);
+    });
+}
+
+clc_status clc_segmented_sort_get_table(clc_context ctx, clc_type key_type,
+                                        int with_values, int* classes)
+{
+    return guarded([&] {
+        require(ctx != nullptr && classes != nullptr, "clc: null argument");
+        const std::vector<int> t =
+            clc::detail::segmented_sort_table(get(ctx), key_type, with_values != 0);
+        std::memcpy(classes, t.data(), t.size() * sizeof(int));
+    });
+}
+
+clc_status clc_segmented_sort_graph_create(clc_context ctx, void* keys,
+                                           clc_type key_type, void* values,
+                                           size_t num_items,
+                                           const uint64_t* offsets,
+                                           size_t num_segments, int descending,
+                                           clc_graph* out)
+{
+    if (out == nullptr)
+        return fail(CLC_ERROR_INVALID_ARGUMENT, "clc: null output");
+    *out = nullptr;
+    return guarded([&] {
+        require(ctx != nullptr, "clc: null context");
+        std::unique_ptr<clc_graph_s> g(new clc_graph_s);
+        g->g = clc::detail::segmented_sort_graph(get(ctx), keys, key_type, values,
+                                                 num_items, offsets, num_segments,
+                                                 descending != 0);
+        *out = g.release();
+    });
+}
+
+clc_status clc_graph_launch(clc_graph graph)
+{
+    if (graph == nullptr || graph->g == nullptr)
+        return fail(CLC_ERROR_INVALID_ARGUMENT, "clc: null graph");
+    return guarded([&] { graph->g->launch(); });
+}
+
+void clc_graph_release(clc_graph graph) { delete graph; }
 
 clc_status clc_operator_supported(clc_type type, const char* op,
                                   const char* identity)

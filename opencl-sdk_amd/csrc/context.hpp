@@ -123,6 +123,25 @@ public:
 
     class profiler& profiler() { return profiler_; }
 
+    //! Autotuned launch tables, keyed by the tuner (key type, values, ...).
+    std::map<int, std::vector<int>>& tuned() { return tuned_; }
+
+    //! Temporarily redirects launches and scratch (used while a graph is
+    //! captured on a private stream with graph-owned scratch); returns the
+    //! previous values so the caller can restore them.
+    hipStream_t swap_stream(hipStream_t s)
+    {
+        hipStream_t old = stream_;
+        stream_ = s;
+        return old;
+    }
+    std::shared_ptr<memory_resource> swap_resource(std::shared_ptr<memory_resource> r)
+    {
+        std::shared_ptr<memory_resource> old = resource_;
+        resource_ = std::move(r);
+        return old;
+    }
+
 private:
     int device_;
     hipStream_t stream_;
@@ -134,6 +153,7 @@ private:
     std::shared_ptr<memory_resource> resource_;
     void* pinned_ = nullptr;
     class profiler profiler_;
+    std::map<int, std::vector<int>> tuned_;
 };
 
 //! Brackets one kernel launch with profiler events when profiling is on.

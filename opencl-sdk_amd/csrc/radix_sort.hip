@@ -25,6 +25,7 @@
 // passes is even, so the result ends in the caller's buffers.
 #include "context.hpp"
 #include "device_common.hpp"
+#include "radix_tile.hpp"
 
 #include <algorithm>
 
@@ -32,36 +33,7 @@ namespace clc {
 namespace {
 
 using namespace dev;
-
-constexpr uint32_t RADIX = 256;
-
-template <typename S> struct status_bits
-{
-    static constexpr int SHIFT = sizeof(S) * 8 - 2;
-    static constexpr S AGG = S(1) << SHIFT; // tile's own count published
-    static constexpr S PREFIX = S(2) << SHIFT; // count of tiles 0..t published
-    static constexpr S FLAGS = S(3) << SHIFT;
-    static constexpr S VALUE = ~FLAGS;
-};
-
-//! Lanes of the wave whose digit equals this lane's: 8 ballots, one per
-//! digit bit. m is the lane's bit b sign-extended to all ones or zero, so
-//! ballot ^ m has a 1 exactly at the lanes that differ in bit b; OR over the
-//! bits leaves 0 at the peers.
-__device__ __forceinline__ uint64_t match_digit(uint32_t d)
-{
-    uint32_t diff_lo = 0, diff_hi = 0;
-#pragma unroll
-    for (int b = 0; b < 8; ++b)
-    {
-        const uint32_t m =
-            static_cast<uint32_t>(static_cast<int32_t>(d << (31 - b)) >> 31);
-        const uint64_t bal = __ballot(m != 0);
-        diff_lo |= static_cast<uint32_t>(bal) ^ m;
-        diff_hi |= static_cast<uint32_t>(bal >> 32) ^ m;
-    }
-    return ~((static_cast<uint64_t>(diff_hi) << 32) | diff_lo);
-}
+using namespace radix;
 
 // ---- upfront histogram of every digit place -------------------------------
 
@@ -108,142 +80,6 @@ __global__ __launch_bounds__(256) void digit_scan_kernel(
     __shared__ uint32_t tmp[4];
     const uint32_t i = blockIdx.x * RADIX + threadIdx.x;
     base[i] = block256_exclusive_add<256>(hist[i], tmp);
-}
-
-// ---- one digit place --------------------------------------------------------
-
-//! One tile of one digit place. FULL tiles skip every bounds check; only
-//! the last tile of the array runs the guarded form.
-template <typename K, typename S, int KIND, bool DESC, bool VALUES, int BLOCK,
-          int ITEMS, bool FULL>
-__device__ __forceinline__ void onesweep_tile(
-    const K* __restrict__ keys_in, K* __restrict__ keys_out,
-    const uint32_t* __restrict__ vals_in, uint32_t* __restrict__ vals_out,
-    uint32_t n, uint32_t shift, const uint32_t* __restrict__ digit_base,
-    S* status, uint32_t tile, uint32_t* s_hist, K* s_keys, uint32_t* s_vals,
-    uint32_t* s_gbase, uint32_t* s_tmp)
-{
-    using SB = status_bits<S>;
-    constexpr int W = BLOCK / WAVE;
-    constexpr uint32_t TILE = BLOCK * ITEMS;
-    const uint32_t tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
-    const uint32_t tile_base = tile * TILE;
-    const uint32_t wave_base = tile_base + wave * (WAVE * ITEMS);
-
-    // Wave-striped loads: round r of a wave covers 64 consecutive keys.
-    K k[ITEMS];
-    uint32_t v[VALUES ? ITEMS : 1];
-#pragma unroll
-    for (int r = 0; r < ITEMS; ++r)
-    {
-        const uint32_t idx = wave_base + r * WAVE + lane;
-        if constexpr (FULL)
-        {
-            k[r] = keys_in[idx];
-            if constexpr (VALUES) v[r] = vals_in[idx];
-        }
-        else
-        {
-            k[r] = idx < n ? keys_in[idx] : K(0);
-            if constexpr (VALUES) v[r] = idx < n ? vals_in[idx] : 0u;
-        }
-    }
-
-    // Rank inside the wave: round by round, each key's rank among equal
-    // digits is the per-wave running count plus its lane-order position in
-    // the round. The lowest lane of each digit group advances the counter.
-    uint32_t rank[ITEMS];
-    uint32_t* my_hist = s_hist + wave * RADIX;
-#pragma unroll
-    for (int r = 0; r < ITEMS; ++r)
-    {
-        const uint32_t d = digit_of<K, KIND, DESC>(k[r], shift);
-        uint64_t peers = match_digit(d);
-        bool valid = true;
-        if constexpr (!FULL)
-        {
-            valid = wave_base + r * WAVE + lane < n;
-            peers &= __ballot(valid);
-        }
-        const uint32_t pre = count_below(peers);
-        const uint32_t base = my_hist[d];
-        if (valid && pre == 0) my_hist[d] = base + __popcll(peers);
-        rank[r] = base + pre;
-    }
-    __syncthreads();
-
-    // Per digit (thread d): tile total, wave-exclusive prefixes, publish.
-    uint32_t total = 0;
-    if (tid < RADIX)
-    {
-#pragma unroll
-        for (int w = 0; w < W; ++w)
-        {
-            const uint32_t c = s_hist[w * RADIX + tid];
-            s_hist[w * RADIX + tid] = total;
-            total += c;
-        }
-        S* mine = status + static_cast<size_t>(tile) * RADIX + tid;
-        store_agent(mine, (tile == 0 ? SB::PREFIX : SB::AGG) | S(total));
-    }
-    const uint32_t digit_start = block256_exclusive_add<BLOCK>(total, s_tmp);
-    if (tid < RADIX)
-    {
-#pragma unroll
-        for (int w = 0; w < W; ++w) s_hist[w * RADIX + tid] += digit_start;
-    }
-    __syncthreads();
-
-    // Reorder the tile in LDS: slot = digit start + earlier waves + rank.
-#pragma unroll
-    for (int r = 0; r < ITEMS; ++r)
-    {
-        if (FULL || wave_base + r * WAVE + lane < n)
-        {
-            const uint32_t d = digit_of<K, KIND, DESC>(k[r], shift);
-            const uint32_t slot = my_hist[d] + rank[r];
-            s_keys[slot] = k[r];
-            if constexpr (VALUES) s_vals[slot] = v[r];
-        }
-    }
-
-    // Decoupled look-back (thread d walks digit d's chain backwards).
-    if (tid < RADIX)
-    {
-        S excl = 0;
-        if (tile > 0)
-        {
-            const S* p = status + static_cast<size_t>(tile - 1) * RADIX + tid;
-            for (;;)
-            {
-                const S s = load_agent(p);
-                if ((s & SB::FLAGS) == 0) continue; // not published yet
-                excl += s & SB::VALUE;
-                if (s & SB::PREFIX) break;
-                p -= RADIX;
-            }
-            store_agent(status + static_cast<size_t>(tile) * RADIX + tid,
-                        SB::PREFIX | (excl + S(total)));
-        }
-        s_gbase[tid] = digit_base[tid] + static_cast<uint32_t>(excl) - digit_start;
-    }
-    __syncthreads();
-
-    // Write the reordered tile: consecutive threads take consecutive slots,
-    // which land in contiguous runs of one digit each.
-    const uint32_t count = FULL ? TILE : n - tile_base;
-#pragma unroll
-    for (int j = 0; j < ITEMS; ++j)
-    {
-        const uint32_t i = j * BLOCK + tid;
-        if (FULL || i < count)
-        {
-            const K key = s_keys[i];
-            const uint32_t dst = s_gbase[digit_of<K, KIND, DESC>(key, shift)] + i;
-            keys_out[dst] = key;
-            if constexpr (VALUES) vals_out[dst] = s_vals[i];
-        }
-    }
 }
 
 template <typename K, typename S, int KIND, bool DESC, bool VALUES, int BLOCK,

@@ -1,0 +1,485 @@
+// Kernels and launch sequence of the segmented sort (see segmented_sort.hpp
+// for the design). Included by the per-key-width instantiation files.
+#pragma once
+
+#include "segmented_sort.hpp"
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+
+namespace clc {
+namespace seg {
+namespace {
+
+using namespace dev;
+using namespace radix;
+
+//! Bucket of a segment of m >= 2 keys: ceil(log2 m), capped.
+__device__ __forceinline__ int bucket_of(uint64_t m)
+{
+    const int b = 64 - __clzll(m - 1);
+    return b < NBUCKET ? b : NBUCKET - 1;
+}
+
+//! Files every segment of >= 2 keys under its class. Segments that break
+//! the contract (end < begin, end > num_items) are left untouched, and the
+//! large class never takes more than large_cap segments, so malformed
+//! offsets cannot drive any kernel out of bounds.
+__global__ __launch_bounds__(256) void classify_kernel(
+    const uint64_t* __restrict__ off, uint32_t nseg, uint64_t num_items, table t,
+    uint32_t large_cap, uint32_t* __restrict__ counts,
+    uint32_t* __restrict__ lists, uint32_t list_stride)
+{
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < nseg; i += gridDim.x * 256)
+    {
+        const uint64_t b = off[i], e = off[i + 1];
+        if (e <= b + 1 || e > num_items) continue;
+        const int c = t.cls[bucket_of(e - b)];
+        const uint32_t pos = atomicAdd(&counts[c], 1u);
+        if (c == LARGE && pos >= large_cap) continue;
+        lists[size_t(c) * list_stride + pos] = i;
+    }
+}
+
+//! Persistent: each workgroup takes segments of its class and sorts each
+//! wholly in LDS.
+template <typename K, int KIND, bool DESC, bool VALUES, int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void lds_class_kernel(
+    K* keys, uint32_t* vals, const uint64_t* __restrict__ off,
+    const uint32_t* __restrict__ list, const uint32_t* count, uint32_t* queue)
+{
+    constexpr int W = BLOCK / WAVE;
+    constexpr uint32_t TILE = BLOCK * ITEMS;
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint32_t* s_hist = reinterpret_cast<uint32_t*>(smem);
+    K* s_keys = reinterpret_cast<K*>(smem + W * 1024);
+    uint32_t* s_vals = reinterpret_cast<uint32_t*>(smem + W * 1024 + TILE * sizeof(K));
+    __shared__ uint32_t s_tmp[4];
+    __shared__ uint32_t s_item;
+    const uint32_t cnt = *count;
+    for (;;)
+    {
+        if (threadIdx.x == 0) s_item = atomicAdd(queue, 1u);
+        __syncthreads();
+        const uint32_t it = s_item;
+        __syncthreads();
+        if (it >= cnt) break;
+        const uint32_t sidx = list[it];
+        const uint64_t b = off[sidx];
+        const uint32_t m = static_cast<uint32_t>(off[sidx + 1] - b);
+        lds_sort_segment<K, KIND, DESC, VALUES, BLOCK, ITEMS>(
+            keys + b, VALUES ? vals + b : nullptr, m, s_hist, s_keys, s_vals, s_tmp);
+    }
+}
+
+//! Single block: lays out the onesweep tiles of the large segments.
+//! Also clamps the large class to large_cap segments and max_tiles tiles
+//! (only malformed offsets reach either bound) and publishes the number of
+//! large segments in use.
+__global__ __launch_bounds__(1024) void tile_scan_kernel(
+    const uint64_t* __restrict__ off, const uint32_t* __restrict__ large_list,
+    const uint32_t* large_count, uint32_t large_cap, uint32_t max_tiles,
+    uint32_t* __restrict__ tile_start, uint32_t* large_used, uint32_t* total_tiles)
+{
+    __shared__ uint32_t s_wave[16];
+    const uint32_t tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
+    const uint32_t L = min(*large_count, large_cap);
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < L; base += 1024)
+    {
+        const uint32_t i = base + tid;
+        uint32_t nt = 0;
+        if (i < L)
+        {
+            const uint32_t s = large_list[i];
+            const uint64_t m = off[s + 1] - off[s];
+            nt = static_cast<uint32_t>((m + LARGE_TILE - 1) / LARGE_TILE);
+        }
+        const uint32_t inc = wave_inclusive_add(nt);
+        if (lane == WAVE - 1) s_wave[wave] = inc;
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+        for (int w = 0; w < 16; ++w)
+        {
+            if (w < static_cast<int>(wave)) before += s_wave[w];
+            total += s_wave[w];
+        }
+        if (i < L) tile_start[i] = carry + before + inc - nt;
+        carry += total;
+        __syncthreads();
+    }
+    if (tid == 0)
+    {
+        tile_start[L] = carry;
+        *large_used = L;
+        // Tiles past max_tiles are dropped; a segment cut short this way
+        // only ever scatters within itself.
+        *total_tiles = min(carry, max_tiles);
+    }
+}
+
+//! Large segment j holding global tile t: tile_start[j] <= t < tile_start[j+1].
+__device__ __forceinline__ uint32_t find_segment(const uint32_t* tile_start,
+                                                 uint32_t L, uint32_t t)
+{
+    uint32_t lo = 0, hi = L; // answer in [lo, hi)
+    while (hi - lo > 1)
+    {
+        const uint32_t mid = (lo + hi) / 2;
+        if (tile_start[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+//! Persistent: per large segment, counts of every digit place.
+template <typename K, int KIND, bool DESC>
+__global__ __launch_bounds__(256) void large_hist_kernel(
+    const K* __restrict__ keys, const uint64_t* __restrict__ off,
+    const uint32_t* __restrict__ large_list, const uint32_t* __restrict__ tile_start,
+    const uint32_t* large_used, const uint32_t* total_tiles,
+    uint32_t* __restrict__ hist, uint32_t* queue)
+{
+    constexpr int P = sizeof(K);
+    __shared__ uint32_t s_h[P * RADIX];
+    __shared__ uint32_t s_item, s_j;
+    const uint32_t L = *large_used, T = *total_tiles;
+    for (;;)
+    {
+        if (threadIdx.x == 0)
+        {
+            const uint32_t t = atomicAdd(queue, 1u);
+            s_item = t;
+            if (t < T) s_j = find_segment(tile_start, L, t);
+        }
+        for (uint32_t i = threadIdx.x; i < P * RADIX; i += 256) s_h[i] = 0;
+        __syncthreads();
+        const uint32_t t = s_item;
+        if (t >= T) break;
+        const uint32_t j = s_j;
+        const uint32_t sidx = large_list[j];
+        const uint64_t b = off[sidx];
+        const uint64_t m = off[sidx + 1] - b;
+        const uint64_t lo = uint64_t(t - tile_start[j]) * LARGE_TILE;
+        const uint64_t hi = std::min<uint64_t>(lo + LARGE_TILE, m);
+        for (uint64_t i = lo + threadIdx.x; i < hi; i += 256)
+        {
+            const K k = keys[b + i];
+#pragma unroll
+            for (int p = 0; p < P; ++p)
+                atomicAdd(&s_h[p * RADIX + digit_of<K, KIND, DESC>(k, 8u * p)], 1u);
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < P * RADIX; i += 256)
+            if (s_h[i] != 0) atomicAdd(&hist[size_t(j) * P * RADIX + i], s_h[i]);
+        __syncthreads();
+    }
+}
+
+//! Persistent: exclusive scan of each (large segment, digit place) count row.
+__global__ __launch_bounds__(256) void large_digit_scan_kernel(
+    const uint32_t* __restrict__ hist, uint32_t* __restrict__ base,
+    const uint32_t* large_used, uint32_t P)
+{
+    __shared__ uint32_t tmp[4];
+    const uint32_t rows = *large_used * P;
+    for (uint32_t r = blockIdx.x; r < rows; r += gridDim.x)
+    {
+        const size_t i = size_t(r) * RADIX + threadIdx.x;
+        base[i] = block256_exclusive_add<256>(hist[i], tmp);
+    }
+}
+
+//! Persistent onesweep over the tiles of every large segment, one digit
+//! place; look-back chains run within a segment.
+template <typename K, int KIND, bool DESC, bool VALUES>
+__global__ __launch_bounds__(LARGE_BLOCK) void large_onesweep_kernel(
+    const K* keys_in, K* keys_out, const uint32_t* vals_in, uint32_t* vals_out,
+    const uint64_t* __restrict__ off, const uint32_t* __restrict__ large_list,
+    const uint32_t* __restrict__ tile_start, const uint32_t* large_used,
+    const uint32_t* total_tiles, uint32_t pass, const uint32_t* __restrict__ base,
+    uint32_t* status, uint32_t* status_next, uint32_t* queue)
+{
+    constexpr int P = sizeof(K);
+    constexpr int W = LARGE_BLOCK / WAVE;
+    __shared__ uint32_t s_hist[W * RADIX];
+    __shared__ K s_keys[LARGE_TILE];
+    __shared__ uint32_t s_vals[VALUES ? LARGE_TILE : 1];
+    __shared__ uint32_t s_gbase[RADIX];
+    __shared__ uint32_t s_tmp[4];
+    __shared__ uint32_t s_item, s_j;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t L = *large_used, T = *total_tiles;
+    for (;;)
+    {
+        if (tid == 0)
+        {
+            const uint32_t t = atomicAdd(queue, 1u);
+            s_item = t;
+            if (t < T) s_j = find_segment(tile_start, L, t);
+        }
+        for (uint32_t i = tid; i < W * RADIX; i += LARGE_BLOCK) s_hist[i] = 0;
+        __syncthreads();
+        const uint32_t t = s_item;
+        if (t >= T) break;
+        const uint32_t j = s_j;
+        if (status_next != nullptr && tid < RADIX) status_next[size_t(t) * RADIX + tid] = 0;
+        const uint32_t sidx = large_list[j];
+        const uint64_t b = off[sidx];
+        const uint32_t m = static_cast<uint32_t>(off[sidx + 1] - b);
+        const uint32_t local = t - tile_start[j];
+        const uint32_t* dbase = base + (size_t(j) * P + pass) * RADIX;
+        uint32_t* st = status + size_t(tile_start[j]) * RADIX;
+        const uint32_t* vi = VALUES ? vals_in + b : nullptr;
+        uint32_t* vo = VALUES ? vals_out + b : nullptr;
+        if (local * LARGE_TILE + LARGE_TILE <= m)
+            onesweep_tile<K, uint32_t, KIND, DESC, VALUES, LARGE_BLOCK, LARGE_ITEMS, true>(
+                keys_in + b, keys_out + b, vi, vo, m, 8u * pass, dbase, st, local,
+                s_hist, s_keys, s_vals, s_gbase, s_tmp);
+        else
+            onesweep_tile<K, uint32_t, KIND, DESC, VALUES, LARGE_BLOCK, LARGE_ITEMS, false>(
+                keys_in + b, keys_out + b, vi, vo, m, 8u * pass, dbase, st, local,
+                s_hist, s_keys, s_vals, s_gbase, s_tmp);
+        __syncthreads();
+    }
+}
+
+struct layout
+{
+    std::size_t bytes = 0;
+    std::size_t take(std::size_t b)
+    {
+        const std::size_t o = bytes;
+        bytes += (b + 255) & ~std::size_t(255);
+        return o;
+    }
+};
+
+//! Resident workgroups of a kernel on the whole device (persistent grid),
+//! cached per kernel.
+template <typename F>
+uint32_t resident_grid(const context& ctx, F kernel, int block, std::size_t lds)
+{
+    static std::mutex m;
+    static std::map<const void*, uint32_t> cache;
+    std::lock_guard<std::mutex> lock(m);
+    const void* key = reinterpret_cast<const void*>(kernel);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, lds)
+            != hipSuccess
+        || per_cu < 1)
+    {
+        (void)hipGetLastError();
+        per_cu = 1;
+    }
+    const uint32_t g =
+        static_cast<uint32_t>(per_cu) * static_cast<uint32_t>(ctx.facts().cu_count);
+    cache[key] = g;
+    return g;
+}
+
+//! Persistent grid of LDS configuration C (also opts the kernel in to more
+//! than 64 KiB of LDS). Computed once per instantiation, outside any
+//! stream capture.
+template <typename K, int KIND, bool DESC, bool VALUES, int C>
+uint32_t lds_class_grid(const context& ctx)
+{
+    constexpr int BLOCK = CFG_BLOCK[C], ITEMS = CFG_ITEMS[C];
+    static uint32_t grid = 0;
+    static std::mutex m;
+    std::lock_guard<std::mutex> lock(m);
+    if (grid == 0)
+    {
+        auto kernel = lds_class_kernel<K, KIND, DESC, VALUES, BLOCK, ITEMS>;
+        const std::size_t lds = cfg_lds(C, sizeof(K), VALUES);
+        if (lds > 65536)
+            CLC_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        static_cast<int>(lds)));
+        grid = resident_grid(ctx, kernel, BLOCK, lds);
+    }
+    return grid;
+}
+
+template <typename K, int KIND, bool DESC, bool VALUES, int C>
+void launch_lds_class(context& ctx, bool prepare, K* keys, uint32_t* vals,
+                      const uint64_t* off, const uint32_t* list, const uint32_t* count,
+                      uint32_t* queue)
+{
+    constexpr int BLOCK = CFG_BLOCK[C], ITEMS = CFG_ITEMS[C];
+    const uint32_t grid = lds_class_grid<K, KIND, DESC, VALUES, C>(ctx);
+    if (prepare) return;
+    timed_launch timer(ctx, "segmented_lds_sort");
+    hipLaunchKernelGGL((lds_class_kernel<K, KIND, DESC, VALUES, BLOCK, ITEMS>), dim3(grid),
+                       dim3(BLOCK), cfg_lds(C, sizeof(K), VALUES), ctx.stream(), keys,
+                       vals, off, list, count, queue);
+}
+
+template <typename K, int KIND, bool DESC, bool VALUES>
+void launch_lds(context& ctx, bool prepare, int c, K* keys, uint32_t* vals,
+                const uint64_t* off, const uint32_t* list, const uint32_t* count,
+                uint32_t* queue)
+{
+#define CLC_LDS_CASE(C)                                                        \
+    case C:                                                                    \
+        launch_lds_class<K, KIND, DESC, VALUES, C>(ctx, prepare, keys, vals,   \
+                                                   off, list, count, queue);   \
+        return;
+    switch (c)
+    {
+        CLC_LDS_CASE(0)
+        CLC_LDS_CASE(1)
+        CLC_LDS_CASE(2)
+        CLC_LDS_CASE(3)
+        CLC_LDS_CASE(4)
+        CLC_LDS_CASE(5)
+        case 6:
+            if constexpr (sizeof(K) == 4)
+            {
+                launch_lds_class<K, KIND, DESC, VALUES, 6>(ctx, prepare, keys, vals,
+                                                           off, list, count, queue);
+                return;
+            }
+            break;
+    }
+#undef CLC_LDS_CASE
+    throw std::invalid_argument("segmented sort: configuration not available for this key type");
+}
+
+template <typename F> uint32_t cached_grid(const context& ctx, F kernel, int block)
+{
+    return resident_grid(ctx, kernel, block, 0);
+}
+
+//! Enqueues the whole segmented sort on ctx's stream. With prepare set it
+//! only computes the scratch size and the persistent grids (so a later
+//! stream capture needs no runtime queries) and returns the size.
+template <typename K, int KIND, bool DESC, bool VALUES>
+std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
+                      std::size_t num_items, const uint64_t* off, std::size_t nseg,
+                      const table& t)
+{
+    constexpr int P = sizeof(K);
+    hipStream_t s = ctx.stream();
+    int first_large = NBUCKET;
+    bool used[NCFG] = {};
+    for (int b = 1; b < NBUCKET; ++b)
+    {
+        if (t.cls[b] == LARGE) first_large = std::min(first_large, b);
+        else if (t.cls[b] >= 0 && t.cls[b] < NCFG) used[t.cls[b]] = true;
+        else throw std::invalid_argument("segmented sort: bad size-class table");
+    }
+    const std::size_t m_min_large =
+        first_large < NBUCKET ? (std::size_t(1) << (first_large - 1)) + 1 : 0;
+    const std::size_t L_max =
+        first_large < NBUCKET ? std::min(nseg, num_items / m_min_large) : 0;
+    const std::size_t T_max = L_max ? (num_items + LARGE_TILE - 1) / LARGE_TILE + L_max : 0;
+
+    layout Lt;
+    const std::size_t o_ctr = Lt.take(256);
+    const std::size_t o_stA = Lt.take(T_max * RADIX * 4);
+    const std::size_t o_hist = Lt.take(L_max * P * RADIX * 4);
+    const std::size_t zero_bytes = Lt.bytes;
+    const std::size_t o_lists = Lt.take((NCFG + 1) * nseg * 4);
+    const std::size_t o_tstart = Lt.take((L_max + 1) * 4);
+    const std::size_t o_base = Lt.take(L_max * P * RADIX * 4);
+    const std::size_t o_stB = Lt.take(T_max * RADIX * 4);
+    const std::size_t o_kalt = L_max ? Lt.take(num_items * sizeof(K)) : 0;
+    const std::size_t o_valt = L_max && VALUES ? Lt.take(num_items * 4) : 0;
+
+    auto hk = large_hist_kernel<K, KIND, DESC>;
+    auto ok = large_onesweep_kernel<K, KIND, DESC, VALUES>;
+    const uint32_t hgrid = L_max ? cached_grid(ctx, hk, 256) : 0;
+    const uint32_t ogrid = L_max ? cached_grid(ctx, ok, LARGE_BLOCK) : 0;
+    if (prepare)
+    {
+        for (int c = 0; c < NCFG; ++c)
+            if (used[c])
+                launch_lds<K, KIND, DESC, VALUES>(ctx, true, c, nullptr, nullptr,
+                                                  nullptr, nullptr, nullptr, nullptr);
+        return Lt.bytes;
+    }
+
+    temp_buffer tmp(ctx.resource(), Lt.bytes);
+    char* base_ptr = tmp.as<char>();
+    uint32_t* ctr = reinterpret_cast<uint32_t*>(base_ptr + o_ctr);
+    // counters: [0, NCFG] list counts, [8, 8 + NCFG) LDS-class queues,
+    // 16 histogram queue, 17 tiles in use, 18 large segments in use,
+    // [24, 24 + P) onesweep queues
+    uint32_t* counts = ctr;
+    uint32_t* lds_q = ctr + 8;
+    uint32_t* hist_q = ctr + 16;
+    uint32_t* total_tiles = ctr + 17;
+    uint32_t* large_used = ctr + 18;
+    uint32_t* pass_q = ctr + 24;
+    uint32_t* lists = reinterpret_cast<uint32_t*>(base_ptr + o_lists);
+    uint32_t* tstart = reinterpret_cast<uint32_t*>(base_ptr + o_tstart);
+    uint32_t* hist = reinterpret_cast<uint32_t*>(base_ptr + o_hist);
+    uint32_t* dbase = reinterpret_cast<uint32_t*>(base_ptr + o_base);
+    uint32_t* st[2] = { reinterpret_cast<uint32_t*>(base_ptr + o_stA),
+                        reinterpret_cast<uint32_t*>(base_ptr + o_stB) };
+
+    detail::zero_fill(s, base_ptr, zero_bytes);
+    const uint32_t cgrid = static_cast<uint32_t>(
+        std::max<std::size_t>(1, std::min<std::size_t>((nseg + 255) / 256, 1024)));
+    hipLaunchKernelGGL(classify_kernel, dim3(cgrid), dim3(256), 0, s, off,
+                       static_cast<uint32_t>(nseg), static_cast<uint64_t>(num_items), t,
+                       static_cast<uint32_t>(L_max), counts, lists,
+                       static_cast<uint32_t>(nseg));
+    for (int c = 0; c < NCFG; ++c)
+        if (used[c])
+            launch_lds<K, KIND, DESC, VALUES>(ctx, false, c, keys, vals, off,
+                                              lists + std::size_t(c) * nseg,
+                                              counts + c, lds_q + c);
+    if (L_max > 0)
+    {
+        const uint32_t* llist = lists + std::size_t(LARGE) * nseg;
+        hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, s, off, llist,
+                           counts + LARGE, static_cast<uint32_t>(L_max),
+                           static_cast<uint32_t>(T_max), tstart, large_used, total_tiles);
+        hipLaunchKernelGGL(hk, dim3(hgrid), dim3(256), 0, s, keys, off, llist, tstart,
+                           large_used, total_tiles, hist, hist_q);
+        const uint32_t sgrid = static_cast<uint32_t>(std::min<std::size_t>(L_max * P, 4096));
+        hipLaunchKernelGGL(large_digit_scan_kernel, dim3(sgrid), dim3(256), 0, s, hist,
+                           dbase, large_used, uint32_t(P));
+        K* kb[2] = { keys, reinterpret_cast<K*>(base_ptr + o_kalt) };
+        uint32_t* vb[2] = { vals, VALUES ? reinterpret_cast<uint32_t*>(base_ptr + o_valt)
+                                         : nullptr };
+        for (int p = 0; p < P; ++p)
+        {
+            const int src = p % 2, dst = 1 - src;
+            timed_launch timer(ctx, "segmented_onesweep");
+            hipLaunchKernelGGL(ok, dim3(ogrid), dim3(LARGE_BLOCK), 0, s, kb[src], kb[dst],
+                               vb[src], vb[dst], off, llist, tstart, large_used,
+                               total_tiles, uint32_t(p), dbase, st[p % 2],
+                               p + 1 < P ? st[(p + 1) % 2] : nullptr, pass_q + p);
+        }
+    }
+    CLC_HIP(hipGetLastError());
+    return Lt.bytes;
+}
+
+} // namespace
+
+template <typename K, int KIND>
+std::size_t sort_typed(context& ctx, bool prepare, void* keys, bool with_values,
+                       void* values, std::size_t num_items, const uint64_t* offsets,
+                       std::size_t num_segments, bool descending, const table& t)
+{
+    K* k = static_cast<K*>(keys);
+    uint32_t* v = static_cast<uint32_t*>(values);
+    const std::size_t n = num_items, ns = num_segments;
+    if (descending)
+    {
+        if (with_values) return sort_impl<K, KIND, true, true>(ctx, prepare, k, v, n, offsets, ns, t);
+        return sort_impl<K, KIND, true, false>(ctx, prepare, k, v, n, offsets, ns, t);
+    }
+    if (with_values) return sort_impl<K, KIND, false, true>(ctx, prepare, k, v, n, offsets, ns, t);
+    return sort_impl<K, KIND, false, false>(ctx, prepare, k, v, n, offsets, ns, t);
+}
+
+} // namespace seg
+} // namespace clc

@@ -1,0 +1,385 @@
+// Host side of the segmented sort: size-class tables (default and
+// autotuned, with an on-disk cache), the entry point, and the hipGraph
+// executor that captures the whole launch sequence once and replays it.
+#include "segmented_sort.hpp"
+
+#include "ops.hpp"
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <sstream>
+#include <unistd.h>
+
+namespace clc {
+namespace seg {
+
+table default_table(std::size_t kbytes, bool values, std::size_t lds_limit)
+{
+    table t;
+    for (int b = 0; b < NBUCKET; ++b)
+    {
+        t.cls[b] = LARGE;
+        // Smallest LDS configuration that holds the whole bucket.
+        for (int c = 0; c < NCFG; ++c)
+        {
+            if ((std::size_t(1) << b) > cfg_capacity(c)) continue;
+            if (kbytes == 8 && c == NCFG - 1) continue; // no 64-bit instantiation
+            if (cfg_lds(c, kbytes, values) > lds_limit) continue;
+            t.cls[b] = c;
+            break;
+        }
+    }
+    return t;
+}
+
+namespace {
+
+using sort_fn = std::size_t (*)(context&, bool, void*, bool, void*, std::size_t,
+                                const uint64_t*, std::size_t, bool, const table&);
+
+sort_fn pick(int key_type)
+{
+    switch (key_type)
+    {
+        case TY_U32: return &sort_typed<uint32_t, dev::KEY_UNSIGNED>;
+        case TY_I32: return &sort_typed<uint32_t, dev::KEY_SIGNED>;
+        case TY_F32: return &sort_typed<uint32_t, dev::KEY_FLOAT>;
+        case TY_U64: return &sort_typed<uint64_t, dev::KEY_UNSIGNED>;
+        case TY_I64: return &sort_typed<uint64_t, dev::KEY_SIGNED>;
+        case TY_F64: return &sort_typed<uint64_t, dev::KEY_FLOAT>;
+    }
+    throw std::invalid_argument(
+        "segmented sort: keys must be I32, U32, F32, I64, U64 or F64");
+}
+
+std::size_t key_bytes(int key_type)
+{
+    return (key_type == TY_U64 || key_type == TY_I64 || key_type == TY_F64) ? 8 : 4;
+}
+
+std::size_t lds_limit(const context& ctx)
+{
+    const std::string& a = ctx.facts().arch;
+    return a.compare(0, 5, "gfx95") == 0 ? 160 * 1024 : 64 * 1024;
+}
+
+int tune_slot(int key_type, bool values) { return 1000 + key_type * 2 + (values ? 1 : 0); }
+
+// ---- on-disk cache of tuned tables ------------------------------------------
+// One text file per (device, key type, values): a header naming the format,
+// the device architecture, CU count and HIP runtime version, then the 32
+// bucket classes. A file whose header does not match this process is
+// ignored (and re-tuned); writes go through a temporary file and rename, so
+// a reader never sees a partial file.
+
+std::string identity_line(const context& ctx)
+{
+    int rt = 0;
+    (void)hipRuntimeGetVersion(&rt);
+    std::ostringstream s;
+    s << "clc-segsort-table-v1 " << ctx.facts().arch << " cu" << ctx.facts().cu_count
+      << " hip" << rt;
+    return s.str();
+}
+
+std::string cache_path(const context& ctx, int key_type, bool values)
+{
+    if (ctx.cache_dir().empty()) return std::string();
+    std::string arch = ctx.facts().arch;
+    for (char& c : arch)
+        if (c == ':' || c == '/' || c == ' ') c = '_';
+    return ctx.cache_dir() + "/clc_segsort_" + arch + "_t" + std::to_string(key_type)
+        + (values ? "_kv" : "_k") + ".txt";
+}
+
+bool load_table(const context& ctx, int key_type, bool values, table& t)
+{
+    const std::string path = cache_path(ctx, key_type, values);
+    if (path.empty()) return false;
+    std::ifstream f(path);
+    if (!f) return false;
+    std::string header;
+    std::getline(f, header);
+    if (header != identity_line(ctx)) return false;
+    table r;
+    for (int b = 0; b < NBUCKET; ++b)
+    {
+        if (!(f >> r.cls[b])) return false;
+        if (r.cls[b] < 0 || r.cls[b] > LARGE) return false;
+        if (r.cls[b] < NCFG
+            && (cfg_lds(r.cls[b], key_bytes(key_type), values) > lds_limit(ctx)
+                || (key_bytes(key_type) == 8 && r.cls[b] == NCFG - 1)
+                || (b > 0 && (std::size_t(1) << b) > cfg_capacity(r.cls[b]))))
+            return false;
+    }
+    t = r;
+    return true;
+}
+
+void store_table(const context& ctx, int key_type, bool values, const table& t)
+{
+    const std::string path = cache_path(ctx, key_type, values);
+    if (path.empty()) return;
+    const std::string tmp = path + ".tmp." + std::to_string(::getpid());
+    {
+        std::ofstream f(tmp, std::ios::trunc);
+        if (!f) return; // the cache is an optimisation: never fail the call
+        f << identity_line(ctx) << "\n";
+        for (int b = 0; b < NBUCKET; ++b) f << t.cls[b] << (b + 1 < NBUCKET ? " " : "\n");
+        if (!f) return;
+    }
+    if (std::rename(tmp.c_str(), path.c_str()) != 0) std::remove(tmp.c_str());
+}
+
+table table_for(context& ctx, int key_type, bool values)
+{
+    auto& tuned = ctx.tuned();
+    const int slot = tune_slot(key_type, values);
+    auto it = tuned.find(slot);
+    table t;
+    if (it != tuned.end() && it->second.size() == NBUCKET)
+    {
+        for (int b = 0; b < NBUCKET; ++b) t.cls[b] = it->second[b];
+        return t;
+    }
+    if (!load_table(ctx, key_type, values, t))
+        t = default_table(key_bytes(key_type), values, lds_limit(ctx));
+    tuned[slot].assign(t.cls, t.cls + NBUCKET);
+    return t;
+}
+
+// ---- autotuner ------------------------------------------------------------------
+
+__global__ void tune_fill_kernel(uint32_t* words, std::size_t n, uint64_t seed)
+{
+    for (std::size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n;
+         i += size_t(gridDim.x) * 256)
+    {
+        // splitmix64 finaliser of (seed, index)
+        uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        words[i] = static_cast<uint32_t>(z ^ (z >> 31));
+    }
+}
+
+//! Times every feasible class for each size bucket on synthetic segments of
+//! that bucket and keeps the fastest.
+table run_tuner(context& ctx, int key_type, bool values)
+{
+    const std::size_t kb = key_bytes(key_type);
+    const sort_fn fn = pick(key_type);
+    const std::size_t limit = lds_limit(ctx);
+    table best = default_table(kb, values, limit);
+    constexpr std::size_t TOTAL = std::size_t(1) << 22;
+    constexpr int MAX_BUCKET = 22;
+    temp_buffer keys(ctx.resource(), TOTAL * kb);
+    temp_buffer vals(ctx.resource(), values ? TOTAL * 4 : 0);
+    temp_buffer offs(ctx.resource(), (TOTAL + 1) * 8);
+    std::vector<uint64_t> host_off;
+    hipEvent_t e0, e1;
+    CLC_HIP(hipEventCreate(&e0));
+    CLC_HIP(hipEventCreate(&e1));
+    struct events
+    {
+        hipEvent_t a, b;
+        ~events()
+        {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } guard{ e0, e1 };
+    hipStream_t s = ctx.stream();
+    const bool profiling = ctx.profiler().enabled();
+    ctx.profiler().enable(false);
+    for (int b = 1; b <= MAX_BUCKET; ++b)
+    {
+        const std::size_t m = b == 1 ? 2 : (std::size_t(3) << (b - 2)); // in (2^(b-1), 2^b]
+        const std::size_t nseg = std::max<std::size_t>(1, TOTAL / m);
+        const std::size_t n = nseg * m;
+        host_off.resize(nseg + 1);
+        for (std::size_t i = 0; i <= nseg; ++i) host_off[i] = i * m;
+        CLC_HIP(hipMemcpyAsync(offs.get(), host_off.data(), (nseg + 1) * 8,
+                               hipMemcpyHostToDevice, s));
+        std::vector<int> cands;
+        for (int c = 0; c < NCFG; ++c)
+            if (cfg_capacity(c) >= (std::size_t(1) << b) && cfg_lds(c, kb, values) <= limit
+                && !(kb == 8 && c == NCFG - 1))
+                cands.push_back(c);
+        if (b >= 12) cands.push_back(LARGE);
+        float best_ms = 1e30f;
+        for (int c : cands)
+        {
+            table t = best;
+            t.cls[b] = c;
+            float run_best = 1e30f;
+            for (int rep = 0; rep < 4; ++rep)
+            {
+                hipLaunchKernelGGL(tune_fill_kernel, dim3(1024), dim3(256), 0, s,
+                                   keys.as<uint32_t>(), n * kb / 4,
+                                   uint64_t(b * 131 + rep));
+                CLC_HIP(hipEventRecord(e0, s));
+                fn(ctx, false, keys.get(), values, vals.get(), n,
+                   offs.as<const uint64_t>(), nseg, false, t);
+                CLC_HIP(hipEventRecord(e1, s));
+                CLC_HIP(hipEventSynchronize(e1));
+                float ms = 0;
+                CLC_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (rep > 0) run_best = std::min(run_best, ms); // rep 0 warms up
+            }
+            if (run_best < best_ms)
+            {
+                best_ms = run_best;
+                best.cls[b] = c;
+            }
+        }
+    }
+    for (int b = MAX_BUCKET + 1; b < NBUCKET; ++b) best.cls[b] = LARGE;
+    ctx.profiler().enable(profiling);
+    CLC_HIP(hipStreamSynchronize(s));
+    return best;
+}
+
+//! Memory resource over one preallocated block: the scratch of a captured
+//! graph, which must outlive the capture and never be recycled.
+class fixed_resource final : public memory_resource {
+public:
+    fixed_resource(void* p, std::size_t bytes) : p_(p), bytes_(bytes) {}
+
+protected:
+    void* do_allocate(std::size_t bytes) override
+    {
+        if (used_ || bytes > bytes_)
+            throw std::logic_error("segmented sort graph: scratch already taken");
+        used_ = true;
+        return p_;
+    }
+    void do_deallocate(void*, std::size_t) noexcept override {}
+
+private:
+    void* p_;
+    std::size_t bytes_;
+    bool used_ = false;
+};
+
+} // namespace
+
+// ---- graph executor ---------------------------------------------------------------
+
+graph::~graph()
+{
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (g) (void)hipGraphDestroy(g);
+    if (scratch) (void)hipFree(scratch);
+    if (capture_stream) (void)hipStreamDestroy(capture_stream);
+}
+
+void graph::launch()
+{
+    ctx->activate();
+    CLC_HIP(hipGraphLaunch(exec, ctx->stream()));
+}
+
+} // namespace seg
+
+namespace detail {
+
+namespace {
+void check_args(int key_type, const void* keys, const uint64_t* offsets,
+                std::size_t num_items, std::size_t num_segments)
+{
+    seg::pick(key_type);
+    if (num_segments > 0xFFFFFFFEu)
+        throw std::invalid_argument("segmented sort: at most 2^32-2 segments");
+    if (num_segments > 0 && (keys == nullptr || offsets == nullptr))
+        throw std::invalid_argument("segmented sort: null keys or offsets");
+    (void)num_items;
+}
+} // namespace
+
+void segmented_sort(context& ctx, void* keys, int key_type, void* values,
+                    std::size_t num_items, const uint64_t* offsets,
+                    std::size_t num_segments, bool descending)
+{
+    check_args(key_type, keys, offsets, num_items, num_segments);
+    if (num_segments == 0 || num_items < 2) return;
+    ctx.activate();
+    const seg::table t = seg::table_for(ctx, key_type, values != nullptr);
+    seg::pick(key_type)(ctx, false, keys, values != nullptr, values, num_items, offsets,
+                        num_segments, descending, t);
+}
+
+void segmented_sort_tune(context& ctx, int key_type, bool with_values)
+{
+    seg::pick(key_type);
+    ctx.activate();
+    const seg::table t = seg::run_tuner(ctx, key_type, with_values);
+    ctx.tuned()[seg::tune_slot(key_type, with_values)].assign(t.cls, t.cls + seg::NBUCKET);
+    seg::store_table(ctx, key_type, with_values, t);
+}
+
+std::vector<int> segmented_sort_table(context& ctx, int key_type, bool with_values)
+{
+    seg::pick(key_type);
+    ctx.activate();
+    const seg::table t = seg::table_for(ctx, key_type, with_values);
+    return std::vector<int>(t.cls, t.cls + seg::NBUCKET);
+}
+
+std::unique_ptr<seg::graph> segmented_sort_graph(context& ctx, void* keys, int key_type,
+                                                 void* values, std::size_t num_items,
+                                                 const uint64_t* offsets,
+                                                 std::size_t num_segments, bool descending)
+{
+    check_args(key_type, keys, offsets, num_items, num_segments);
+    ctx.activate();
+    const seg::table t = seg::table_for(ctx, key_type, values != nullptr);
+    const seg::sort_fn fn = seg::pick(key_type);
+    std::unique_ptr<seg::graph> g(new seg::graph);
+    g->ctx = &ctx;
+    const std::size_t bytes = fn(ctx, true, keys, values != nullptr, values, num_items,
+                                 offsets, num_segments, descending, t);
+    CLC_HIP(hipMalloc(&g->scratch, bytes));
+    CLC_HIP(hipStreamCreateWithFlags(&g->capture_stream, hipStreamNonBlocking));
+    // Launch everything on a private stream under capture, drawing scratch
+    // from the graph's own block and with event profiling off.
+    const hipStream_t old_stream = ctx.swap_stream(g->capture_stream);
+    auto old_res = ctx.swap_resource(std::make_shared<seg::fixed_resource>(g->scratch, bytes));
+    const bool profiling = ctx.profiler().enabled();
+    ctx.profiler().enable(false);
+    struct restore
+    {
+        context& c;
+        hipStream_t s;
+        std::shared_ptr<memory_resource> r;
+        bool p;
+        ~restore()
+        {
+            c.swap_stream(s);
+            c.swap_resource(r);
+            c.profiler().enable(p);
+        }
+    } guard{ ctx, old_stream, old_res, profiling };
+    CLC_HIP(hipStreamBeginCapture(g->capture_stream, hipStreamCaptureModeThreadLocal));
+    try
+    {
+        if (num_segments > 0 && num_items >= 2)
+            fn(ctx, false, keys, values != nullptr, values, num_items, offsets,
+               num_segments, descending, t);
+    } catch (...)
+    {
+        hipGraph_t broken = nullptr;
+        (void)hipStreamEndCapture(g->capture_stream, &broken);
+        if (broken) (void)hipGraphDestroy(broken);
+        throw;
+    }
+    CLC_HIP(hipStreamEndCapture(g->capture_stream, &g->g));
+    CLC_HIP(hipGraphInstantiate(&g->exec, g->g, nullptr, nullptr, 0));
+    return g;
+}
+
+} // namespace detail
+} // namespace clc

@@ -1,0 +1,111 @@
+// Segmented (batched) stable radix sort for gfx950: thousands of uneven
+// segments sorted in one fixed-shape launch sequence that needs no host
+// round trip and can be captured into a hipGraph.
+//
+//   classify      each segment goes to a size class by the tuned table
+//                 (bucket = ceil(log2 m)): one of the LDS configurations
+//                 below, or "large"; per-class lists are filled by atomics
+//   lds classes   one persistent kernel per configuration; each workgroup
+//                 takes segments from its class list and sorts each one
+//                 wholly in LDS (lds_sort_segment): one read and one write
+//                 of HBM per key
+//   large         segments above every LDS capacity: a single-block scan
+//                 lays out their onesweep tiles, one persistent kernel
+//                 counts every digit place per segment, one scans the
+//                 counts, and one persistent onesweep kernel per digit place
+//                 processes all large segments' tiles, with look-back chains
+//                 per segment
+//
+// The reference library has no segmented sort; its per-segment equivalent
+// is one detail::radix_sort call per segment (Algorithm.cpp:453-527).
+#pragma once
+
+#include "context.hpp"
+#include "device_common.hpp"
+#include "radix_tile.hpp"
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace clc {
+namespace seg {
+
+//! LDS configurations (workgroup size, items per thread); a segment of m
+//! keys fits configuration c when m <= BLOCK * ITEMS.
+constexpr int NCFG = 7;
+constexpr int CFG_BLOCK[NCFG] = { 64, 64, 128, 256, 256, 512, 1024 };
+constexpr int CFG_ITEMS[NCFG] = { 4, 8, 8, 8, 16, 16, 16 };
+//! Class index of the large (onesweep) path.
+constexpr int LARGE = NCFG;
+//! Segment-size buckets: bucket b holds 2^(b-1) < m <= 2^b (m >= 2).
+constexpr int NBUCKET = 32;
+//! Onesweep tile of the large path.
+constexpr int LARGE_BLOCK = 256, LARGE_ITEMS = 16;
+constexpr uint32_t LARGE_TILE = LARGE_BLOCK * LARGE_ITEMS;
+
+inline std::size_t cfg_capacity(int c)
+{
+    return std::size_t(CFG_BLOCK[c]) * CFG_ITEMS[c];
+}
+
+//! Dynamic LDS bytes of configuration c for key size kbytes.
+inline std::size_t cfg_lds(int c, std::size_t kbytes, bool values)
+{
+    const std::size_t tile = cfg_capacity(c);
+    return std::size_t(CFG_BLOCK[c] / 64) * 1024 + tile * kbytes
+        + (values ? tile * 4 : 0) + 64;
+}
+
+//! Class per bucket; the autotuner fills it, defaults come from
+//! default_table().
+struct table
+{
+    int cls[NBUCKET];
+};
+
+table default_table(std::size_t kbytes, bool values, std::size_t lds_limit);
+
+//! Enqueues the segmented sort for one key type on ctx's stream, or with
+//! prepare set only sizes its scratch and warms its launch state. Returns
+//! the scratch bytes. Instantiated in segmented_k32.hip / segmented_k64.hip.
+template <typename K, int KIND>
+std::size_t sort_typed(context& ctx, bool prepare, void* keys, bool with_values,
+                       void* values, std::size_t num_items, const uint64_t* offsets,
+                       std::size_t num_segments, bool descending, const table& t);
+
+//! A captured segmented sort: the whole launch sequence as one hipGraph,
+//! with scratch owned by the graph, replayed on the context's stream.
+struct graph
+{
+    context* ctx = nullptr;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void* scratch = nullptr;
+    hipStream_t capture_stream = nullptr;
+    ~graph();
+    void launch();
+};
+
+} // namespace seg
+
+namespace detail {
+
+//! Stable sort of every segment [offsets[i], offsets[i+1]) of keys/values.
+void segmented_sort(context& ctx, void* keys, int key_type, void* values,
+                    std::size_t num_items, const uint64_t* offsets,
+                    std::size_t num_segments, bool descending);
+//! Runs the autotuner for a key type and stores its table in ctx (and on
+//! disk when ctx has a cache directory).
+void segmented_sort_tune(context& ctx, int key_type, bool with_values);
+//! The size-class table the segmented sort will use.
+std::vector<int> segmented_sort_table(context& ctx, int key_type, bool with_values);
+//! Captures the segmented sort of fixed buffers into a graph.
+std::unique_ptr<seg::graph> segmented_sort_graph(context& ctx, void* keys, int key_type,
+                                                 void* values, std::size_t num_items,
+                                                 const uint64_t* offsets,
+                                                 std::size_t num_segments, bool descending);
+
+} // namespace detail
+} // namespace clc

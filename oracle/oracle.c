@@ -517,3 +517,64 @@ int oracle_histogram_even(int type, const void* in, size_t n, uint32_t* hist,
     }
     return 0;
 }
+
+/* ---- segmented sort (no counterpart in the reference) -------------------- */
+
+typedef struct
+{
+    int key_type, desc, id, nthreads;
+    char* keys;
+    uint32_t* values;
+    size_t kbytes;
+    const uint64_t* off;
+    size_t nseg;
+    int rc;
+} seg_job;
+
+static void* seg_worker(void* p)
+{
+    seg_job* j = (seg_job*)p;
+    for (size_t i = (size_t)j->id; i < j->nseg; i += (size_t)j->nthreads)
+    {
+        const uint64_t b = j->off[i], e = j->off[i + 1];
+        if (e <= b + 1) continue;
+        int rc = oracle_radix_sort(j->key_type, j->keys + b * j->kbytes,
+                                   j->values ? j->values + b : NULL, (size_t)(e - b),
+                                   j->desc, 1);
+        if (rc) j->rc = rc;
+    }
+    return NULL;
+}
+
+/* Every segment [off[i], off[i+1]) sorted on its own by the restated LSD
+ * sort above: the per-segment loop of clc_sort calls a caller of the
+ * reference would write. Segments with end <= begin + 1 are left as they
+ * are. */
+int oracle_segmented_sort(int key_type, void* keys, uint32_t* values,
+                          const uint64_t* off, size_t nseg, int descending,
+                          int threads)
+{
+    size_t kbytes;
+    if (key_type == T_U32 || key_type == T_I32 || key_type == T_F32) kbytes = 4;
+    else if (key_type == T_U64 || key_type == T_I64 || key_type == T_F64) kbytes = 8;
+    else return -1;
+    if (threads <= 0) threads = 1;
+    if (threads > MAX_THREADS) threads = MAX_THREADS;
+    seg_job jobs[MAX_THREADS];
+    pthread_t tid[MAX_THREADS];
+    for (int t = 0; t < threads; ++t)
+    {
+        seg_job jb = { key_type, descending != 0, t, threads, (char*)keys, values,
+                       kbytes, off, nseg, 0 };
+        jobs[t] = jb;
+        if (t > 0) pthread_create(&tid[t], NULL, seg_worker, &jobs[t]);
+    }
+    seg_worker(&jobs[0]);
+    int rc = jobs[0].rc;
+    for (int t = 1; t < threads; ++t)
+    {
+        pthread_join(tid[t], NULL);
+        if (jobs[t].rc) rc = jobs[t].rc;
+    }
+    return rc;
+}

@@ -63,6 +63,9 @@ def lib() -> ctypes.CDLL:
         L.oracle_histogram_even.argtypes = [ctypes.c_int, vp, sz, vp, sz,
                                             ctypes.c_double, ctypes.c_double]
         L.oracle_histogram_even.restype = ctypes.c_int
+        L.oracle_segmented_sort.argtypes = [ctypes.c_int, vp, vp, vp, sz, ctypes.c_int,
+                                            ctypes.c_int]
+        L.oracle_segmented_sort.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -138,6 +141,31 @@ def radix_sort(keys: np.ndarray, values: np.ndarray | None = None,
     if rc != 0:
         raise MemoryError("oracle_radix_sort: host allocation failed")
     return k, v
+
+
+def segmented_sort(keys: np.ndarray, values, offsets: np.ndarray,
+                   descending: bool = False, threads: int = 1):
+    """Each segment [offsets[i], offsets[i+1]) sorted by the restated LSD
+    sort on its own. Returns sorted copies (keys, values-or-None)."""
+    k = np.ascontiguousarray(keys).copy()
+    v = None if values is None else np.ascontiguousarray(values, dtype=np.uint32).copy()
+    off = np.ascontiguousarray(offsets, dtype=np.uint64)
+    rc = lib().oracle_segmented_sort(TYPE_CODES[k.dtype], _ptr(k), _ptr(v), _ptr(off),
+                                     off.size - 1, 1 if descending else 0, threads)
+    if rc != 0:
+        raise ValueError("oracle_segmented_sort failed")
+    return k, v
+
+
+def lognormal_segments(nseg: int, mu: float, sigma: float, lo: int, hi: int,
+                       seed: int) -> np.ndarray:
+    """Offsets (nseg + 1, uint64) of segments with sizes
+    round(exp(N(mu, sigma))) clamped to [lo, hi]."""
+    rng = np.random.default_rng(seed)
+    sizes = np.clip(np.rint(np.exp(rng.normal(mu, sigma, nseg))), lo, hi).astype(np.uint64)
+    off = np.zeros(nseg + 1, dtype=np.uint64)
+    np.cumsum(sizes, out=off[1:])
+    return off
 
 
 def scan(x: np.ndarray, op: str = "plus", identity=0, exclusive: bool = False) -> np.ndarray:

@@ -200,6 +200,35 @@ def test_histogram_reference_test_vectors(golden, n):
                                   golden[f"hist_expect_{n}"])
 
 
+@pytest.mark.parametrize("dtype", [np.uint32, np.float32, np.int64])
+@pytest.mark.parametrize("descending", [False, True])
+def test_segmented_sort_matches_per_segment_numpy(dtype, descending):
+    rng = np.random.default_rng(21)
+    off = oracle.lognormal_segments(300, np.log(40), 1.5, 0, 5000, seed=4)
+    n = int(off[-1])
+    if np.dtype(dtype).kind == "f":
+        keys = rng.standard_normal(n).astype(dtype)
+        keys[::7] = 0.0
+        keys[::11] = -0.0
+    else:
+        keys = rng.integers(-50, 50, size=n).astype(dtype)
+    vals = np.arange(n, dtype=np.uint32)
+    k, v = oracle.segmented_sort(keys, vals, off, descending, threads=4)
+    for i in range(off.size - 1):
+        b, e = int(off[i]), int(off[i + 1])
+        perm = np.argsort(oracle.transformed_keys(keys[b:e], descending), kind="stable")
+        np.testing.assert_array_equal(v[b:e], vals[b:e][perm])
+        np.testing.assert_array_equal(k[b:e].view(np.uint8), keys[b:e][perm].view(np.uint8))
+
+
+def test_lognormal_segments_shape():
+    off = oracle.lognormal_segments(65536, np.log(512), 1.5, 1, 1 << 20, seed=1)
+    sizes = np.diff(off)
+    assert sizes.min() >= 1 and sizes.max() <= (1 << 20)
+    assert 400 <= np.median(sizes) <= 650          # median exp(mu) = 512
+    assert 2**25 < off[-1] < 2**27                  # ~ 65536 * 512 * e^1.125
+
+
 def test_histogram_brute_force_float_bins():
     rng = np.random.default_rng(9)
     x = (rng.standard_normal(50000) * 3).astype(np.float32)

@@ -141,10 +141,13 @@ def test_scan_full_size_property(ctx):
     out = torch.empty_like(x)
     ctx.scan(x, out, "(a + b)", "0", exclusive=True, type_code=clc_hip.U32)
     torch.cuda.synchronize()
-    o = out.to(torch.int64)
-    assert int(o[0]) == 0
-    assert bool(((o[1:] - o[:-1]) == x[:-1].to(torch.int64)).all())
-    assert int(o[-1]) + int(x[-1]) == int(x.to(torch.int64).sum())
+    # u32 arithmetic wraps modulo 2^32; compare the differences modulo 2^32.
+    mask = 0xFFFFFFFF
+    o = out.to(torch.int64) & mask
+    first_ok = int(o[0]) == 0
+    diffs_ok = bool((((o[1:] - o[:-1]) & mask) == x[:-1].to(torch.int64)).all())
+    total_ok = (int(o[-1]) + int(x[-1])) & mask == int(x.to(torch.int64).sum()) & mask
+    assert first_ok and diffs_ok and total_ok
 
 
 def test_scan_unaligned(ctx):

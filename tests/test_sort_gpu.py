@@ -203,14 +203,18 @@ def check_stable_sort_properties(orig_keys, keys, vals):
     if keys.dtype == torch.int64:
         # unsigned 64-bit order via flipping the sign bit
         k = keys ^ torch.tensor(-(1 << 63), dtype=torch.int64, device=keys.device)
-    assert bool((k[1:] >= k[:-1]).all())
+    # Predicates are reduced to Python bools before assert, so a failure
+    # never makes pytest render 2^28-element tensors.
+    ascending = bool((k[1:] >= k[:-1]).all())
     v = vals.to(torch.int64) & 0xFFFFFFFF
     seen = torch.zeros(n, dtype=torch.bool, device=keys.device)
     seen[v] = True
-    assert bool(seen.all())
-    assert bool((orig_keys[v] == keys).all())
+    permutation = bool(seen.all())
+    consistent = bool((orig_keys[v] == keys).all())
     ties = k[1:] == k[:-1]
-    assert bool((v[1:][ties] > v[:-1][ties]).all())
+    stable = bool((v[1:][ties] > v[:-1][ties]).all())
+    assert ascending and permutation and consistent and stable, \
+        (ascending, permutation, consistent, stable)
 
 
 def test_full_size_2p28_pairs(ctx):
