@@ -112,12 +112,12 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
         status_next[static_cast<size_t>(tile) * RADIX + tid] = 0;
 
     if (tile * TILE + TILE <= n)
-        onesweep_tile<K, S, KIND, DESC, VALUES, BLOCK, ITEMS, true>(
-            keys_in, keys_out, vals_in, vals_out, n, shift, digit_base, status,
+        onesweep_tile<K, S, VALUES, BLOCK, ITEMS, true>(
+            keys_in, keys_out, vals_in, vals_out, n, key_digit<K, KIND, DESC>{ shift }, digit_base, status,
             tile, s_hist, s_keys, s_vals, s_gbase, s_tmp);
     else
-        onesweep_tile<K, S, KIND, DESC, VALUES, BLOCK, ITEMS, false>(
-            keys_in, keys_out, vals_in, vals_out, n, shift, digit_base, status,
+        onesweep_tile<K, S, VALUES, BLOCK, ITEMS, false>(
+            keys_in, keys_out, vals_in, vals_out, n, key_digit<K, KIND, DESC>{ shift }, digit_base, status,
             tile, s_hist, s_keys, s_vals, s_gbase, s_tmp);
 }
 

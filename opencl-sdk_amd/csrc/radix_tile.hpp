@@ -41,16 +41,26 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d)
     return ~((static_cast<uint64_t>(diff_hi) << 32) | diff_lo);
 }
 
+//! Digit of a key for one pass: byte shift/8 of the key's order image.
+template <typename K, int KIND, bool DESC> struct key_digit
+{
+    uint32_t shift;
+    __device__ __forceinline__ uint32_t operator()(K k) const
+    {
+        return digit_of<K, KIND, DESC>(k, shift);
+    }
+};
+
 // ---- one digit place --------------------------------------------------------
 
 //! One tile of one digit place. FULL tiles skip every bounds check; only
 //! the last tile of the array runs the guarded form.
-template <typename K, typename S, int KIND, bool DESC, bool VALUES, int BLOCK,
-          int ITEMS, bool FULL>
+template <typename K, typename S, bool VALUES, int BLOCK, int ITEMS, bool FULL,
+          typename DF>
 __device__ __forceinline__ void onesweep_tile(
     const K* __restrict__ keys_in, K* __restrict__ keys_out,
     const uint32_t* __restrict__ vals_in, uint32_t* __restrict__ vals_out,
-    uint32_t n, uint32_t shift, const uint32_t* __restrict__ digit_base,
+    uint32_t n, DF dig, const uint32_t* __restrict__ digit_base,
     S* status, uint32_t tile, uint32_t* s_hist, K* s_keys, uint32_t* s_vals,
     uint32_t* s_gbase, uint32_t* s_tmp)
 {
@@ -88,7 +98,7 @@ __device__ __forceinline__ void onesweep_tile(
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r)
     {
-        const uint32_t d = digit_of<K, KIND, DESC>(k[r], shift);
+        const uint32_t d = dig(k[r]);
         uint64_t peers = match_digit(d);
         bool valid = true;
         if constexpr (!FULL)
@@ -131,7 +141,7 @@ __device__ __forceinline__ void onesweep_tile(
     {
         if (FULL || wave_base + r * WAVE + lane < n)
         {
-            const uint32_t d = digit_of<K, KIND, DESC>(k[r], shift);
+            const uint32_t d = dig(k[r]);
             const uint32_t slot = my_hist[d] + rank[r];
             s_keys[slot] = k[r];
             if constexpr (VALUES) s_vals[slot] = v[r];
@@ -170,7 +180,7 @@ __device__ __forceinline__ void onesweep_tile(
         if (FULL || i < count)
         {
             const K key = s_keys[i];
-            const uint32_t dst = s_gbase[digit_of<K, KIND, DESC>(key, shift)] + i;
+            const uint32_t dst = s_gbase[dig(key)] + i;
             keys_out[dst] = key;
             if constexpr (VALUES) vals_out[dst] = s_vals[i];
         }
