@@ -183,6 +183,15 @@ clc_status clc_scan(clc_context ctx, const void* in, void* out, size_t n,
 clc_status clc_sort(clc_context ctx, void* keys, clc_type key_type,
                     void* values, size_t n, int descending);
 
+/* Times every onesweep tile shape on 2^26 synthetic keys of key_type (with
+ * or without values) and keeps the fastest for later clc_sort calls on ctx
+ * with that key width; persisted like clc_segmented_sort_tune. *shape (may
+ * be NULL) receives the chosen shape index. Synchronous. No counterpart in
+ * the reference, whose work-group size comes from the device limit
+ * (MemoryResource.cpp:187-197). */
+clc_status clc_sort_tune(clc_context ctx, clc_type key_type, int with_values,
+                         int* shape);
+
 /* Zeroes num_bins uint32 counters, then counts the samples with
  * lower <= x < upper into bin (uint)((x - lower) * scale), scale =
  * bins / (upper - lower), computed in float (double for F64 samples) and

@@ -82,6 +82,7 @@ SIGNATURES = {
     "clc_histogram_even": (_i, [_vp, _vp, _sz, _i, _vp, _sz, ctypes.c_double,
                                 ctypes.c_double]),
     "clc_operator_supported": (_i, [_i, _cp, _cp]),
+    "clc_sort_tune": (_i, [_vp, _i, _i, ctypes.POINTER(_i)]),
     "clc_segmented_sort": (_i, [_vp, _vp, _i, _vp, _sz, _vp, _sz, _i]),
     "clc_segmented_sort_tune": (_i, [_vp, _i, _i]),
     "clc_segmented_sort_get_table": (_i, [_vp, _i, _i, ctypes.POINTER(ctypes.c_int)]),
@@ -218,6 +219,12 @@ class Context:
             self.handle, ctypes.c_void_p(keys.data_ptr()), kt,
             None if values is None else ctypes.c_void_p(values.data_ptr()), n,
             ctypes.c_void_p(offsets.data_ptr()), ns, 1 if descending else 0))
+
+    def sort_tune(self, key_type, with_values=True) -> int:
+        shape = ctypes.c_int(-1)
+        check(lib().clc_sort_tune(self.handle, key_type, 1 if with_values else 0,
+                                  ctypes.byref(shape)))
+        return shape.value
 
     def segmented_sort_tune(self, key_type, with_values=True):
         check(lib().clc_segmented_sort_tune(self.handle, key_type, 1 if with_values else 0))

@@ -343,6 +343,16 @@ clc_status clc_sort(clc_context ctx, void* keys, clc_type key_type,
     });
 }
 
+clc_status clc_sort_tune(clc_context ctx, clc_type key_type, int with_values,
+                         int* shape)
+{
+    return guarded([&] {
+        require(ctx != nullptr, "clc: null context");
+        const int s = clc::detail::sort_tune(get(ctx), key_type, with_values != 0);
+        if (shape != nullptr) *shape = s;
+    });
+}
+
 clc_status clc_histogram_even(clc_context ctx, const void* in, size_t n,
                               clc_type type, uint32_t* histogram,
                               size_t num_bins, double lower, double upper)

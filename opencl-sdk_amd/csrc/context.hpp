@@ -174,6 +174,21 @@ private:
     int token_;
 };
 
+// ---- on-disk cache of tuned launch tables ----------------------------------
+// One text file per table name in the context's cache directory: a header
+// naming the format, device architecture, CU count and HIP runtime version,
+// then the integers. A file whose header does not match this process is
+// ignored (and re-tuned). Writes go to a temporary file renamed into place,
+// so readers never see a partial file. Mirrors the reference's on-disk
+// program cache (lib/src/Compute/ProgramCache.cpp:163-219).
+
+//! Reads table name; false when there is no cache directory, no file, or a
+//! stale/foreign one.
+bool load_tuning(const context& ctx, const std::string& name, std::vector<int>& out);
+//! Writes table name; failures are ignored (the cache is an optimisation).
+void store_tuning(const context& ctx, const std::string& name,
+                  const std::vector<int>& values);
+
 //! Kernels index sizes up to 2^31-1 per call, as the reference
 //! (lib/src/Compute/Detail.hpp:18-23).
 inline void check_size(std::size_t n)
@@ -198,6 +213,11 @@ void histogram_even(context& ctx, const void* in, std::size_t n, int type,
 //! Zeroes bytes of device memory with a kernel (graph-capturable, unlike a
 //! memset node).
 void zero_fill(hipStream_t stream, void* p, std::size_t bytes);
+//! Times every onesweep tile shape on 2^26 synthetic keys of key_type and
+//! keeps the fastest for this key width and values flag; returns it.
+int sort_tune(context& ctx, int key_type, bool with_values);
+int sort_tuning_slot(std::size_t kbytes, bool values);
+std::string sort_tuning_name(std::size_t kbytes, bool values);
 
 } // namespace detail
 

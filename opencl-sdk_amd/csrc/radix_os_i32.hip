@@ -1,0 +1,10 @@
+// Onesweep pass instantiations for i32 keys (every tile shape, direction,
+// values flag and status-word width).
+#include "radix_onesweep.hpp"
+
+namespace clc {
+namespace radix {
+template void launch_onesweep<uint32_t, dev::KEY_SIGNED>(int, bool, bool, bool, const onesweep_args&,
+                                           hipStream_t);
+} // namespace radix
+} // namespace clc

@@ -25,9 +25,14 @@
 // passes is even, so the result ends in the caller's buffers.
 #include "context.hpp"
 #include "device_common.hpp"
+#include "ops.hpp"
+#include "radix_onesweep.hpp"
 #include "radix_tile.hpp"
 
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
 
 namespace clc {
 namespace {
@@ -82,45 +87,6 @@ __global__ __launch_bounds__(256) void digit_scan_kernel(
     base[i] = block256_exclusive_add<256>(hist[i], tmp);
 }
 
-template <typename K, typename S, int KIND, bool DESC, bool VALUES, int BLOCK,
-          int ITEMS>
-__global__ __launch_bounds__(BLOCK) void onesweep_kernel(
-    const K* __restrict__ keys_in, K* __restrict__ keys_out,
-    const uint32_t* __restrict__ vals_in, uint32_t* __restrict__ vals_out,
-    uint32_t n, uint32_t shift, const uint32_t* __restrict__ digit_base,
-    S* status, S* status_next, uint32_t* tile_counter)
-{
-    static_assert(BLOCK >= 256 && BLOCK % WAVE == 0, "block shape");
-    constexpr int W = BLOCK / WAVE;
-    constexpr uint32_t TILE = BLOCK * ITEMS;
-
-    __shared__ uint32_t s_hist[W * RADIX]; // per-wave digit counters
-    __shared__ K s_keys[TILE]; // tile reordered by digit
-    __shared__ uint32_t s_vals[VALUES ? TILE : 1];
-    __shared__ uint32_t s_gbase[RADIX]; // global destination - tile digit start
-    __shared__ uint32_t s_tmp[4];
-    __shared__ uint32_t s_tile;
-
-    const uint32_t tid = threadIdx.x;
-    if (tid == 0) s_tile = fetch_add_agent(tile_counter, 1u);
-    for (uint32_t i = tid; i < W * RADIX; i += BLOCK) s_hist[i] = 0;
-    __syncthreads();
-    const uint32_t tile = s_tile;
-    // Clear this tile's status words for the next digit place; that pass
-    // starts after this one has finished, so nobody reads them meanwhile.
-    if (status_next != nullptr && tid < RADIX)
-        status_next[static_cast<size_t>(tile) * RADIX + tid] = 0;
-
-    if (tile * TILE + TILE <= n)
-        onesweep_tile<K, S, VALUES, BLOCK, ITEMS, true>(
-            keys_in, keys_out, vals_in, vals_out, n, key_digit<K, KIND, DESC>{ shift }, digit_base, status,
-            tile, s_hist, s_keys, s_vals, s_gbase, s_tmp);
-    else
-        onesweep_tile<K, S, VALUES, BLOCK, ITEMS, false>(
-            keys_in, keys_out, vals_in, vals_out, n, key_digit<K, KIND, DESC>{ shift }, digit_base, status,
-            tile, s_hist, s_keys, s_vals, s_gbase, s_tmp);
-}
-
 struct layout
 {
     std::size_t bytes = 0;
@@ -132,33 +98,57 @@ struct layout
     }
 };
 
-template <typename K, typename S, int KIND, bool DESC, bool VALUES>
-void sort_impl(context& ctx, K* keys, uint32_t* vals, uint32_t n)
+//! Tile shape for a key width and values flag: the CLC_ONESWEEP_CFG
+//! environment variable, else the tuned shape, else shape 0.
+int pick_shape(context& ctx, std::size_t kbytes, bool values)
+{
+    if (const char* e = std::getenv("CLC_ONESWEEP_CFG"))
+    {
+        const int c = std::atoi(e);
+        if (c >= 0 && c < NOS) return c;
+    }
+    auto& tuned = ctx.tuned();
+    auto it = tuned.find(detail::sort_tuning_slot(kbytes, values));
+    if (it != tuned.end() && it->second.size() == 1 && it->second[0] >= 0
+        && it->second[0] < NOS)
+        return it->second[0];
+    std::vector<int> stored;
+    if (load_tuning(ctx, detail::sort_tuning_name(kbytes, values), stored)
+        && stored.size() == 1 && stored[0] >= 0 && stored[0] < NOS)
+    {
+        tuned[detail::sort_tuning_slot(kbytes, values)] = stored;
+        return stored[0];
+    }
+    return 0;
+}
+
+template <typename K, int KIND>
+void sort_impl(context& ctx, K* keys, uint32_t* vals, uint32_t n, bool desc, int shape)
 {
     constexpr int P = sizeof(K);
-    constexpr int BLOCK = 256, ITEMS = 16;
-    constexpr uint32_t TILE = BLOCK * ITEMS;
-    const uint32_t tiles = (n + TILE - 1) / TILE;
+    const bool values = vals != nullptr;
+    const bool wide = n >= (1u << 30); // 30-bit counts fit a 32-bit status word
+    const std::size_t sbytes = wide ? 8 : 4;
+    const uint32_t tiles = (n + os_tile(shape) - 1) / os_tile(shape);
     hipStream_t s = ctx.stream();
 
     layout L;
     const std::size_t o_hist = L.take(P * RADIX * 4);
     const std::size_t o_ctr = L.take(P * 4);
-    const std::size_t o_st0 = L.take(std::size_t(tiles) * RADIX * sizeof(S));
+    const std::size_t o_st0 = L.take(std::size_t(tiles) * RADIX * sbytes);
     const std::size_t zero_bytes = L.bytes;
-    const std::size_t o_st1 = L.take(std::size_t(tiles) * RADIX * sizeof(S));
+    const std::size_t o_st1 = L.take(std::size_t(tiles) * RADIX * sbytes);
     const std::size_t o_base = L.take(P * RADIX * 4);
     const std::size_t o_kalt = L.take(std::size_t(n) * sizeof(K));
-    const std::size_t o_valt = VALUES ? L.take(std::size_t(n) * 4) : 0;
+    const std::size_t o_valt = values ? L.take(std::size_t(n) * 4) : 0;
     temp_buffer tmp(ctx.resource(), L.bytes);
     char* t = tmp.as<char>();
     uint32_t* hist = reinterpret_cast<uint32_t*>(t + o_hist);
     uint32_t* ctr = reinterpret_cast<uint32_t*>(t + o_ctr);
-    S* st[2] = { reinterpret_cast<S*>(t + o_st0), reinterpret_cast<S*>(t + o_st1) };
+    void* st[2] = { t + o_st0, t + o_st1 };
     uint32_t* base = reinterpret_cast<uint32_t*>(t + o_base);
     K* kb[2] = { keys, reinterpret_cast<K*>(t + o_kalt) };
-    uint32_t* vb[2] = { vals, VALUES ? reinterpret_cast<uint32_t*>(t + o_valt)
-                                     : nullptr };
+    uint32_t* vb[2] = { vals, values ? reinterpret_cast<uint32_t*>(t + o_valt) : nullptr };
 
     detail::zero_fill(s, t, zero_bytes);
     const uint32_t hist_grid =
@@ -166,76 +156,144 @@ void sort_impl(context& ctx, K* keys, uint32_t* vals, uint32_t n)
                                         uint32_t(ctx.facts().cu_count) * 8));
     {
         timed_launch timer(ctx, "radix_digit_histogram");
-        if (reinterpret_cast<uintptr_t>(keys) % 16 == 0)
-            hipLaunchKernelGGL((digit_histogram_kernel<K, KIND, DESC, 256, true>),
-                               dim3(hist_grid), dim3(256), 0, s, keys, n, hist);
-        else
-            hipLaunchKernelGGL((digit_histogram_kernel<K, KIND, DESC, 256, false>),
-                               dim3(hist_grid), dim3(256), 0, s, keys, n, hist);
+        const bool vec = reinterpret_cast<uintptr_t>(keys) % 16 == 0;
+#define CLC_HIST(D, V)                                                          \
+    hipLaunchKernelGGL((digit_histogram_kernel<K, KIND, D, 256, V>), dim3(hist_grid), \
+                       dim3(256), 0, s, keys, n, hist)
+        if (desc) { if (vec) CLC_HIST(true, true); else CLC_HIST(true, false); }
+        else { if (vec) CLC_HIST(false, true); else CLC_HIST(false, false); }
+#undef CLC_HIST
     }
     hipLaunchKernelGGL(digit_scan_kernel, dim3(P), dim3(256), 0, s, hist, base);
     for (int p = 0; p < P; ++p)
     {
         const int src = p % 2, dst = 1 - src;
-        timed_launch timer(ctx, VALUES ? "radix_onesweep_pairs" : "radix_onesweep_keys");
-        hipLaunchKernelGGL(
-            (onesweep_kernel<K, S, KIND, DESC, VALUES, BLOCK, ITEMS>),
-            dim3(tiles), dim3(BLOCK), 0, s, kb[src], kb[dst], vb[src], vb[dst],
-            n, uint32_t(8 * p), base + p * RADIX, st[p % 2],
-            p + 1 < P ? st[(p + 1) % 2] : nullptr, ctr + p);
+        timed_launch timer(ctx, values ? "radix_onesweep_pairs" : "radix_onesweep_keys");
+        onesweep_args a{ kb[src], kb[dst], vb[src], vb[dst], n, uint32_t(8 * p),
+                         base + p * RADIX, st[p % 2],
+                         p + 1 < P ? st[(p + 1) % 2] : nullptr, ctr + p };
+        launch_onesweep<K, KIND>(shape, desc, values, wide, a, s);
     }
     CLC_HIP(hipGetLastError());
 }
 
+using sort_fn = void (*)(context&, void*, void*, uint32_t, bool, int);
+
 template <typename K, int KIND>
-void sort_dispatch(context& ctx, void* keys, void* values, uint32_t n, bool desc)
+void sort_entry(context& ctx, void* keys, void* values, uint32_t n, bool desc, int shape)
 {
-    K* k = static_cast<K*>(keys);
-    uint32_t* v = static_cast<uint32_t*>(values);
-    const bool wide = n >= (1u << 30);
-#define CLC_SORT_CASE(S, D, V)                                                 \
-    sort_impl<K, S, KIND, D, V>(ctx, k, v, n)
-    if (wide)
+    sort_impl<K, KIND>(ctx, static_cast<K*>(keys), static_cast<uint32_t*>(values), n,
+                       desc, shape);
+}
+
+sort_fn pick_sort(int key_type)
+{
+    switch (key_type)
     {
-        if (desc) { if (v) CLC_SORT_CASE(uint64_t, true, true); else CLC_SORT_CASE(uint64_t, true, false); }
-        else { if (v) CLC_SORT_CASE(uint64_t, false, true); else CLC_SORT_CASE(uint64_t, false, false); }
+        case TY_U32: return &sort_entry<uint32_t, KEY_UNSIGNED>;
+        case TY_I32: return &sort_entry<uint32_t, KEY_SIGNED>;
+        case TY_F32: return &sort_entry<uint32_t, KEY_FLOAT>;
+        case TY_U64: return &sort_entry<uint64_t, KEY_UNSIGNED>;
+        case TY_I64: return &sort_entry<uint64_t, KEY_SIGNED>;
+        case TY_F64: return &sort_entry<uint64_t, KEY_FLOAT>;
     }
-    else
+    throw std::invalid_argument("clc_sort: keys must be I32, U32, F32, I64, U64 or F64");
+}
+
+std::size_t key_bytes_of(int key_type)
+{
+    return (key_type == TY_U64 || key_type == TY_I64 || key_type == TY_F64) ? 8 : 4;
+}
+
+__global__ void fill_words_kernel(uint32_t* w, std::size_t n, uint64_t seed)
+{
+    for (std::size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n;
+         i += size_t(gridDim.x) * 256)
     {
-        if (desc) { if (v) CLC_SORT_CASE(uint32_t, true, true); else CLC_SORT_CASE(uint32_t, true, false); }
-        else { if (v) CLC_SORT_CASE(uint32_t, false, true); else CLC_SORT_CASE(uint32_t, false, false); }
+        uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ull; // splitmix64
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        w[i] = static_cast<uint32_t>(z ^ (z >> 31));
     }
-#undef CLC_SORT_CASE
 }
 
 } // namespace
 
 namespace detail {
 
+int sort_tuning_slot(std::size_t kbytes, bool values)
+{
+    return 2000 + static_cast<int>(kbytes) * 2 + (values ? 1 : 0);
+}
+
+std::string sort_tuning_name(std::size_t kbytes, bool values)
+{
+    return "sort_k" + std::to_string(kbytes * 8) + (values ? "_kv" : "_k");
+}
+
 void radix_sort(context& ctx, void* keys, int key_type, void* values,
                 std::size_t n, bool descending)
 {
-    switch (key_type)
-    {
-        case 4: case 5: case 8: case 6: case 7: case 9: break;
-        default:
-            throw std::invalid_argument(
-                "clc_sort: keys must be I32, U32, F32, I64, U64 or F64");
-    }
+    const sort_fn fn = pick_sort(key_type);
     if (n <= 1) return;
     check_size(n);
     if (keys == nullptr) throw std::invalid_argument("clc_sort: null keys");
     ctx.activate();
-    const uint32_t m = static_cast<uint32_t>(n);
-    switch (key_type)
+    const int shape = pick_shape(ctx, key_bytes_of(key_type), values != nullptr);
+    fn(ctx, keys, values, static_cast<uint32_t>(n), descending, shape);
+}
+
+int sort_tune(context& ctx, int key_type, bool with_values)
+{
+    const sort_fn fn = pick_sort(key_type);
+    ctx.activate();
+    const std::size_t kb = key_bytes_of(key_type);
+    constexpr uint32_t N = 1u << 26; // 512 MiB of pairs: past the 256 MB MALL
+    temp_buffer keys(ctx.resource(), std::size_t(N) * kb);
+    temp_buffer vals(ctx.resource(), with_values ? std::size_t(N) * 4 : 0);
+    hipEvent_t e0, e1;
+    CLC_HIP(hipEventCreate(&e0));
+    CLC_HIP(hipEventCreate(&e1));
+    struct guard
     {
-        case 5: sort_dispatch<uint32_t, KEY_UNSIGNED>(ctx, keys, values, m, descending); break;
-        case 4: sort_dispatch<uint32_t, KEY_SIGNED>(ctx, keys, values, m, descending); break;
-        case 8: sort_dispatch<uint32_t, KEY_FLOAT>(ctx, keys, values, m, descending); break;
-        case 7: sort_dispatch<uint64_t, KEY_UNSIGNED>(ctx, keys, values, m, descending); break;
-        case 6: sort_dispatch<uint64_t, KEY_SIGNED>(ctx, keys, values, m, descending); break;
-        case 9: sort_dispatch<uint64_t, KEY_FLOAT>(ctx, keys, values, m, descending); break;
+        hipEvent_t a, b;
+        ~guard()
+        {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } g{ e0, e1 };
+    hipStream_t s = ctx.stream();
+    const bool profiling = ctx.profiler().enabled();
+    ctx.profiler().enable(false);
+    int best = 0;
+    float best_ms = 1e30f;
+    for (int shape = 0; shape < NOS; ++shape)
+    {
+        float shape_ms = 1e30f;
+        for (int rep = 0; rep < 4; ++rep)
+        {
+            hipLaunchKernelGGL(fill_words_kernel, dim3(2048), dim3(256), 0, s,
+                               keys.as<uint32_t>(), std::size_t(N) * kb / 4,
+                               uint64_t(rep + 1));
+            CLC_HIP(hipEventRecord(e0, s));
+            fn(ctx, keys.get(), with_values ? vals.get() : nullptr, N, false, shape);
+            CLC_HIP(hipEventRecord(e1, s));
+            CLC_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            CLC_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (rep > 0) shape_ms = std::min(shape_ms, ms); // rep 0 warms up
+        }
+        if (shape_ms < best_ms)
+        {
+            best_ms = shape_ms;
+            best = shape;
+        }
     }
+    ctx.profiler().enable(profiling);
+    ctx.tuned()[sort_tuning_slot(kb, with_values)] = { best };
+    store_tuning(ctx, sort_tuning_name(kb, with_values), { best });
+    return best;
 }
 
 } // namespace detail

@@ -158,7 +158,11 @@ __device__ __forceinline__ void onesweep_tile(
             for (;;)
             {
                 const S s = load_agent(p);
-                if ((s & SB::FLAGS) == 0) continue; // not published yet
+                if ((s & SB::FLAGS) == 0) // not published yet: back off briefly
+                {
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
                 excl += s & SB::VALUE;
                 if (s & SB::PREFIX) break;
                 p -= RADIX;

@@ -68,46 +68,24 @@ std::size_t lds_limit(const context& ctx)
 
 int tune_slot(int key_type, bool values) { return 1000 + key_type * 2 + (values ? 1 : 0); }
 
-// ---- on-disk cache of tuned tables ------------------------------------------
-// One text file per (device, key type, values): a header naming the format,
-// the device architecture, CU count and HIP runtime version, then the 32
-// bucket classes. A file whose header does not match this process is
-// ignored (and re-tuned); writes go through a temporary file and rename, so
-// a reader never sees a partial file.
+// Tuned tables persist through the context's tuning cache (load_tuning /
+// store_tuning), one entry per key type and values flag; an entry that names
+// a class infeasible for this key type is treated as absent.
 
-std::string identity_line(const context& ctx)
+std::string table_name(int key_type, bool values)
 {
-    int rt = 0;
-    (void)hipRuntimeGetVersion(&rt);
-    std::ostringstream s;
-    s << "clc-segsort-table-v1 " << ctx.facts().arch << " cu" << ctx.facts().cu_count
-      << " hip" << rt;
-    return s.str();
-}
-
-std::string cache_path(const context& ctx, int key_type, bool values)
-{
-    if (ctx.cache_dir().empty()) return std::string();
-    std::string arch = ctx.facts().arch;
-    for (char& c : arch)
-        if (c == ':' || c == '/' || c == ' ') c = '_';
-    return ctx.cache_dir() + "/clc_segsort_" + arch + "_t" + std::to_string(key_type)
-        + (values ? "_kv" : "_k") + ".txt";
+    return "segsort_t" + std::to_string(key_type) + (values ? "_kv" : "_k");
 }
 
 bool load_table(const context& ctx, int key_type, bool values, table& t)
 {
-    const std::string path = cache_path(ctx, key_type, values);
-    if (path.empty()) return false;
-    std::ifstream f(path);
-    if (!f) return false;
-    std::string header;
-    std::getline(f, header);
-    if (header != identity_line(ctx)) return false;
+    std::vector<int> v;
+    if (!load_tuning(ctx, table_name(key_type, values), v) || v.size() != NBUCKET)
+        return false;
     table r;
     for (int b = 0; b < NBUCKET; ++b)
     {
-        if (!(f >> r.cls[b])) return false;
+        r.cls[b] = v[b];
         if (r.cls[b] < 0 || r.cls[b] > LARGE) return false;
         if (r.cls[b] < NCFG
             && (cfg_lds(r.cls[b], key_bytes(key_type), values) > lds_limit(ctx)
@@ -121,17 +99,8 @@ bool load_table(const context& ctx, int key_type, bool values, table& t)
 
 void store_table(const context& ctx, int key_type, bool values, const table& t)
 {
-    const std::string path = cache_path(ctx, key_type, values);
-    if (path.empty()) return;
-    const std::string tmp = path + ".tmp." + std::to_string(::getpid());
-    {
-        std::ofstream f(tmp, std::ios::trunc);
-        if (!f) return; // the cache is an optimisation: never fail the call
-        f << identity_line(ctx) << "\n";
-        for (int b = 0; b < NBUCKET; ++b) f << t.cls[b] << (b + 1 < NBUCKET ? " " : "\n");
-        if (!f) return;
-    }
-    if (std::rename(tmp.c_str(), path.c_str()) != 0) std::remove(tmp.c_str());
+    store_tuning(ctx, table_name(key_type, values),
+                 std::vector<int>(t.cls, t.cls + NBUCKET));
 }
 
 table table_for(context& ctx, int key_type, bool values)

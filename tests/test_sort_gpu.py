@@ -181,6 +181,35 @@ def test_unaligned_pointer(ctx):
     assert host(kd, np.uint32)[0] == keys[0]
 
 
+@pytest.mark.parametrize("shape", range(6))
+@pytest.mark.parametrize("dtype,descending", [(np.uint32, False), (np.float32, True),
+                                              (np.uint64, False), (np.int64, True)])
+def test_every_tile_shape(ctx, monkeypatch, shape, dtype, descending):
+    """Every onesweep tile shape the tuner can pick gives the oracle's output."""
+    monkeypatch.setenv("CLC_ONESWEEP_CFG", str(shape))
+    for n in (1000, 24577, 300007):
+        say("sort shape", shape, np.dtype(dtype).name, descending, n)
+        keys = make_keys(dtype, n, shape + n)
+        vals = np.arange(n, dtype=np.uint32)
+        k, v = gpu_sort(ctx, keys, vals, descending)
+        ek, ev = oracle.radix_sort(keys, vals, descending=descending, threads=8)
+        assert_same(k, ek)
+        np.testing.assert_array_equal(v, ev)
+
+
+def test_sort_tuner_picks_a_valid_shape(ctx):
+    say("sort tuner u32 pairs")
+    shape = ctx.sort_tune(clc_hip.U32, True)
+    assert 0 <= shape < 6
+    n = 100003
+    keys = oracle.pcg32_u32(77, 1, n)
+    vals = np.arange(n, dtype=np.uint32)
+    k, v = gpu_sort(ctx, keys, vals)
+    ek, ev = oracle.radix_sort(keys, vals)
+    np.testing.assert_array_equal(k, ek)
+    np.testing.assert_array_equal(v, ev)
+
+
 def test_errors(ctx):
     L = clc_hip.lib()
     say("sort error paths")
