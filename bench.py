@@ -57,7 +57,10 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--log2n", type=int, default=28)
+    ap.add_argument("--log2n", type=int, default=None,
+                    help="log2 of the key count (default 28 for u32, 27 for u64)")
+    ap.add_argument("--key", choices=["u32", "u64"], default="u32",
+                    help="u32: full 32-bit keys; u64: keys uniform over [0, 2^48)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-log2n", type=int, default=26)
     ap.add_argument("--cpu-reps", type=int, default=2)
@@ -70,19 +73,27 @@ def main() -> int:
     import torch
     import clc_hip
 
-    n = 1 << args.log2n
+    wide = args.key == "u64"
+    log2n = args.log2n if args.log2n is not None else (27 if wide else 28)
+    n = 1 << log2n
+    kbytes = 8 if wide else 4
+    passes = 8 if wide else 4
+    executed = 6 if wide else 4  # u64 keys below 2^48: the top two bytes are constant
     torch.cuda.set_device(0)
     g = torch.Generator(device="cuda")
     g.manual_seed(42)
-    keys0 = torch.randint(-(1 << 31), (1 << 31) - 1, (n,), dtype=torch.int32,
-                          device="cuda", generator=g)
+    if wide:
+        keys0 = torch.randint(0, 1 << 48, (n,), dtype=torch.int64, device="cuda", generator=g)
+    else:
+        keys0 = torch.randint(-(1 << 31), (1 << 31) - 1, (n,), dtype=torch.int32,
+                              device="cuda", generator=g)
     vals0 = torch.arange(n, dtype=torch.int32, device="cuda")
     keys = torch.empty_like(keys0)
     vals = torch.empty_like(vals0)
     ctx = clc_hip.Context()
 
     def step():
-        ctx.sort(keys, vals, key_type=clc_hip.U32)
+        ctx.sort(keys, vals, key_type=clc_hip.U64 if wide else clc_hip.U32)
 
     for _ in range(args.warmup):
         keys.copy_(keys0)
@@ -111,7 +122,7 @@ def main() -> int:
 
     # Output check on the last step: ascending keys, values a permutation
     # consistent with the keys, ties in index order (stable).
-    k = keys.to(torch.int64) & 0xFFFFFFFF
+    k = keys if wide else keys.to(torch.int64) & 0xFFFFFFFF
     ok = bool((k[1:] >= k[:-1]).all())
     ok = ok and bool((keys0[vals.long()] == keys).all())
     ties = k[1:] == k[:-1]
@@ -120,11 +131,15 @@ def main() -> int:
     one = timings.get("radix_onesweep_pairs")
     roofline = None
     if one and one["launches"]:
-        avg_ms = one["total_ms"] / one["launches"]
-        bytes_per_launch = n * 2 * (4 + 4)  # read + write of key and value
+        # Skipped (constant-digit) passes launch but return at once: average
+        # over the executed launches only.
+        ran = one["launches"] * executed // passes
+        avg_ms = one["total_ms"] / ran
+        bytes_per_launch = n * 2 * (kbytes + 4)  # read + write of key and value
         achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
         traffic = None
-        tpath = os.path.join(ROOT, "profiles", "onesweep_pairs_2p28_traffic.json")
+        tpath = os.path.join(ROOT, "profiles",
+                             f"onesweep_pairs_{args.key}_2p{log2n}_traffic.json")
         if os.path.exists(tpath):
             with open(tpath) as f:
                 traffic = json.load(f).get("hbm_bytes_per_launch")
@@ -133,7 +148,8 @@ def main() -> int:
                     "traffic": traffic, "kernel": "radix_onesweep_pairs",
                     "avg_launch_ms": round(avg_ms, 4),
                     "algorithmic_bytes_per_launch": bytes_per_launch}
-    sort_bytes = n * (4 + 4 * 2 * (4 + 4))  # k + P*2*(k+v), P = 4
+    sort_bytes = n * (kbytes + passes * 2 * (kbytes + 4))  # k + P*2(k+v)
+    executed_bytes = n * (kbytes + executed * 2 * (kbytes + 4))
     value = n / (ms / 1e3) / 1e9
     line = {
         "metric": METRIC,
@@ -146,15 +162,19 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u32",
-        "data": "synthetic (torch.randint seeded, uniform 32-bit keys; values 0..n-1)",
-        "config": {"workload": f"stable key-value radix sort, 2^{args.log2n} uint32 keys "
-                               "uniform over 32 bits + uint32 index values, ascending",
-                   "n": n, "key_bytes": 4, "value_bytes": 4, "passes": 4,
-                   "parallelism": "single GPU"},
+        "dtype": args.key,
+        "data": "synthetic (torch.randint seeded; values 0..n-1)",
+        "config": {"workload": (f"stable key-value radix sort, 2^{log2n} uint64 keys uniform "
+                                "over [0, 2^48) + uint32 index values, ascending") if wide else
+                               (f"stable key-value radix sort, 2^{log2n} uint32 keys "
+                                "uniform over 32 bits + uint32 index values, ascending"),
+                   "n": n, "key_bytes": kbytes, "value_bytes": 4, "passes": passes,
+                   "executed_passes": executed, "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": None,
         "extra": {"sort_roofline_frac": round(sort_bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                  "executed_pass_roofline_frac":
+                      round(executed_bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
                   "step_ms_min": round(min(step_ms), 4),
                   "step_ms_median": round(statistics.median(step_ms), 4),
                   "step_ms_max": round(max(step_ms), 4),

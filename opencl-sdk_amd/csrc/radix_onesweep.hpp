@@ -9,30 +9,39 @@
 namespace clc {
 namespace radix {
 
-//! Tile shapes (workgroup size x keys per thread). The sort tuner picks one
-//! per key width and values flag; shape 0 is the default.
-constexpr int NOS = 6;
-constexpr int OS_BLOCK[NOS] = { 256, 256, 512, 512, 1024, 256 };
-constexpr int OS_ITEMS[NOS] = { 16, 8, 16, 8, 8, 24 };
+//! Tile shapes (workgroup size x keys per thread, split staging). The sort
+//! tuner picks one per key width and values flag; shape 0 is the default.
+constexpr int NOS = 8;
+constexpr int OS_BLOCK[NOS] = { 512, 256, 256, 1024, 512, 512, 512, 1024 };
+constexpr int OS_ITEMS[NOS] = { 16, 16, 24, 8, 16, 24, 32, 16 };
+constexpr bool OS_SPLIT[NOS] = { false, false, false, false, true, true, true, true };
 
 inline uint32_t os_tile(int cfg) { return uint32_t(OS_BLOCK[cfg]) * OS_ITEMS[cfg]; }
 
-//! Dynamic LDS bytes of shape cfg.
+//! Dynamic LDS bytes of shape cfg: per-wave counters, destination bases and
+//! the staging buffer (keys and values together, or one after the other).
 inline std::size_t os_lds(int cfg, std::size_t kbytes, bool values)
 {
     const std::size_t tile = os_tile(cfg);
-    return std::size_t(OS_BLOCK[cfg] / 64) * 1024 + tile * kbytes
-        + (values ? tile * 4 : 0) + RADIX * 4 + 64;
+    const std::size_t stage =
+        OS_SPLIT[cfg] ? tile * kbytes : tile * kbytes + (values ? tile * 4 : 0);
+    return std::size_t(OS_BLOCK[cfg] / 64) * 1024 + RADIX * 4 + stage + 64;
 }
 
+//! One digit place. The keys and values live in two buffers (a = the
+//! caller's, b = scratch); which one the pass reads is decided on the device:
+//! passes whose digit is the same for every key (trivial[p] != 0, set by the
+//! digit scan) are skipped, and each executed pass reads the buffer the
+//! previous executed pass wrote.
 struct onesweep_args
 {
-    const void* keys_in;
-    void* keys_out;
-    const uint32_t* vals_in;
-    uint32_t* vals_out;
+    void* keys_a;
+    void* keys_b;
+    uint32_t* vals_a;
+    uint32_t* vals_b;
     uint32_t n;
-    uint32_t shift;
+    uint32_t pass;
+    const uint32_t* trivial; // one flag per digit place
     const uint32_t* digit_base;
     void* status; // uint32_t words, or uint64_t when wide_status
     void* status_next;
@@ -46,12 +55,12 @@ void launch_onesweep(int cfg, bool descending, bool values, bool wide_status,
                      const onesweep_args& a, hipStream_t s);
 
 template <typename K, typename S, int KIND, bool DESC, bool VALUES, int BLOCK,
-          int ITEMS>
+          int ITEMS, bool SPLIT>
 __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
-    const K* __restrict__ keys_in, K* __restrict__ keys_out,
-    const uint32_t* __restrict__ vals_in, uint32_t* __restrict__ vals_out,
-    uint32_t n, uint32_t shift, const uint32_t* __restrict__ digit_base,
-    S* status, S* status_next, uint32_t* tile_counter)
+    K* keys_a, K* keys_b, uint32_t* vals_a, uint32_t* vals_b, uint32_t n,
+    uint32_t pass, const uint32_t* __restrict__ trivial,
+    const uint32_t* __restrict__ digit_base, S* status, S* status_next,
+    uint32_t* tile_counter)
 {
     static_assert(BLOCK >= 256 && BLOCK % WAVE == 0, "block shape");
     constexpr int W = BLOCK / WAVE;
@@ -65,6 +74,21 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
     __shared__ uint32_t s_tile;
 
     const uint32_t tid = threadIdx.x;
+    if (trivial[pass] != 0)
+    {
+        // Every key has the same digit here: the pass would be the identity.
+        // Only clear the next pass's status words (one row per workgroup).
+        if (status_next != nullptr && tid < RADIX)
+            status_next[static_cast<size_t>(blockIdx.x) * RADIX + tid] = 0;
+        return;
+    }
+    uint32_t parity = 0; // executed passes before this one, mod 2
+    for (uint32_t p = 0; p < pass; ++p) parity ^= trivial[p] == 0 ? 1u : 0u;
+    const K* keys_in = parity ? keys_b : keys_a;
+    K* keys_out = parity ? keys_a : keys_b;
+    const uint32_t* vals_in = parity ? vals_b : vals_a;
+    uint32_t* vals_out = parity ? vals_a : vals_b;
+
     if (tid == 0) s_tile = fetch_add_agent(tile_counter, 1u);
     for (uint32_t i = tid; i < W * RADIX; i += BLOCK) s_hist[i] = 0;
     __syncthreads();
@@ -74,13 +98,13 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
     if (status_next != nullptr && tid < RADIX)
         status_next[static_cast<size_t>(tile) * RADIX + tid] = 0;
 
-    const key_digit<K, KIND, DESC> dig{ shift };
+    const key_digit<K, KIND, DESC> dig{ 8u * pass };
     if (tile * TILE + TILE <= n)
-        onesweep_tile<K, S, VALUES, BLOCK, ITEMS, true>(
+        onesweep_tile<K, S, VALUES, BLOCK, ITEMS, true, SPLIT>(
             keys_in, keys_out, vals_in, vals_out, n, dig, digit_base, status, tile,
             s_hist, s_keys, s_vals, s_gbase, s_tmp);
     else
-        onesweep_tile<K, S, VALUES, BLOCK, ITEMS, false>(
+        onesweep_tile<K, S, VALUES, BLOCK, ITEMS, false, SPLIT>(
             keys_in, keys_out, vals_in, vals_out, n, dig, digit_base, status, tile,
             s_hist, s_keys, s_vals, s_gbase, s_tmp);
 }
@@ -91,7 +115,7 @@ template <typename K, typename S, int KIND, bool DESC, bool VALUES, int C>
 void launch(const onesweep_args& a, hipStream_t s)
 {
     constexpr int BLOCK = OS_BLOCK[C], ITEMS = OS_ITEMS[C];
-    auto kernel = onesweep_kernel<K, S, KIND, DESC, VALUES, BLOCK, ITEMS>;
+    auto kernel = onesweep_kernel<K, S, KIND, DESC, VALUES, BLOCK, ITEMS, OS_SPLIT[C]>;
     const std::size_t lds = os_lds(C, sizeof(K), VALUES);
     static bool opted = false; // benign race: the attribute call is idempotent
     if (!opted && lds > 65536)
@@ -103,8 +127,8 @@ void launch(const onesweep_args& a, hipStream_t s)
     }
     const uint32_t tiles = (a.n + os_tile(C) - 1) / os_tile(C);
     hipLaunchKernelGGL(kernel, dim3(tiles), dim3(BLOCK), lds, s,
-                       static_cast<const K*>(a.keys_in), static_cast<K*>(a.keys_out),
-                       a.vals_in, a.vals_out, a.n, a.shift, a.digit_base,
+                       static_cast<K*>(a.keys_a), static_cast<K*>(a.keys_b), a.vals_a,
+                       a.vals_b, a.n, a.pass, a.trivial, a.digit_base,
                        static_cast<S*>(a.status), static_cast<S*>(a.status_next),
                        a.tile_counter);
 }
@@ -120,6 +144,8 @@ void by_cfg(int cfg, const onesweep_args& a, hipStream_t s)
         case 3: launch<K, S, KIND, DESC, VALUES, 3>(a, s); return;
         case 4: launch<K, S, KIND, DESC, VALUES, 4>(a, s); return;
         case 5: launch<K, S, KIND, DESC, VALUES, 5>(a, s); return;
+        case 6: launch<K, S, KIND, DESC, VALUES, 6>(a, s); return;
+        case 7: launch<K, S, KIND, DESC, VALUES, 7>(a, s); return;
     }
     throw std::invalid_argument("radix sort: unknown tile shape");
 }

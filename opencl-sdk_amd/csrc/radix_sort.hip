@@ -21,8 +21,11 @@
 // only ever waits on tiles that are already running (no co-residency
 // assumption). Ranks follow input order inside a wave (lane order within a
 // 64-key round, rounds in order), across waves (wave-ordered prefixes) and
-// across tiles (look-back), which keeps the sort stable. P = sizeof(K)
-// passes is even, so the result ends in the caller's buffers.
+// across tiles (look-back), which keeps the sort stable. A digit place where
+// every key has the same digit is skipped on the device (its pass would be
+// the identity); each executed pass reads what the previous one wrote, and
+// if an odd number ran, a final copy returns the result to the caller's
+// buffers (in place, as Algorithm.cpp:526). No host round trip is needed.
 #include "context.hpp"
 #include "device_common.hpp"
 #include "ops.hpp"
@@ -79,12 +82,36 @@ __global__ __launch_bounds__(BLOCK) void digit_histogram_kernel(
 }
 
 //! base[p][d] = sum of hist[p][d'] for d' < d; one block per digit place.
+//! A place where one digit holds all n keys is marked trivial (its pass
+//! would be the identity permutation; trivial[] starts zeroed).
 __global__ __launch_bounds__(256) void digit_scan_kernel(
-    const uint32_t* __restrict__ hist, uint32_t* __restrict__ base)
+    const uint32_t* __restrict__ hist, uint32_t* __restrict__ base, uint32_t n,
+    uint32_t* __restrict__ trivial)
 {
     __shared__ uint32_t tmp[4];
     const uint32_t i = blockIdx.x * RADIX + threadIdx.x;
-    base[i] = block256_exclusive_add<256>(hist[i], tmp);
+    const uint32_t c = hist[i];
+    base[i] = block256_exclusive_add<256>(c, tmp);
+    if (c == n) trivial[blockIdx.x] = 1;
+}
+
+//! After the passes: if an odd number of them ran, the result sits in the
+//! scratch buffers b; copy it back to the caller's a.
+template <typename K>
+__global__ __launch_bounds__(256) void finish_copy_kernel(
+    K* __restrict__ keys_a, const K* __restrict__ keys_b, uint32_t* __restrict__ vals_a,
+    const uint32_t* __restrict__ vals_b, uint32_t n, const uint32_t* __restrict__ trivial,
+    uint32_t passes)
+{
+    uint32_t odd = 0;
+    for (uint32_t p = 0; p < passes; ++p) odd ^= trivial[p] == 0 ? 1u : 0u;
+    if (!odd) return;
+    const uint32_t stride = gridDim.x * 256;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
+    {
+        keys_a[i] = keys_b[i];
+        if (vals_a != nullptr) vals_a[i] = vals_b[i];
+    }
 }
 
 struct layout
@@ -135,6 +162,7 @@ void sort_impl(context& ctx, K* keys, uint32_t* vals, uint32_t n, bool desc, int
     layout L;
     const std::size_t o_hist = L.take(P * RADIX * 4);
     const std::size_t o_ctr = L.take(P * 4);
+    const std::size_t o_triv = L.take(P * 4);
     const std::size_t o_st0 = L.take(std::size_t(tiles) * RADIX * sbytes);
     const std::size_t zero_bytes = L.bytes;
     const std::size_t o_st1 = L.take(std::size_t(tiles) * RADIX * sbytes);
@@ -145,6 +173,7 @@ void sort_impl(context& ctx, K* keys, uint32_t* vals, uint32_t n, bool desc, int
     char* t = tmp.as<char>();
     uint32_t* hist = reinterpret_cast<uint32_t*>(t + o_hist);
     uint32_t* ctr = reinterpret_cast<uint32_t*>(t + o_ctr);
+    uint32_t* trivial = reinterpret_cast<uint32_t*>(t + o_triv);
     void* st[2] = { t + o_st0, t + o_st1 };
     uint32_t* base = reinterpret_cast<uint32_t*>(t + o_base);
     K* kb[2] = { keys, reinterpret_cast<K*>(t + o_kalt) };
@@ -164,16 +193,20 @@ void sort_impl(context& ctx, K* keys, uint32_t* vals, uint32_t n, bool desc, int
         else { if (vec) CLC_HIST(false, true); else CLC_HIST(false, false); }
 #undef CLC_HIST
     }
-    hipLaunchKernelGGL(digit_scan_kernel, dim3(P), dim3(256), 0, s, hist, base);
+    hipLaunchKernelGGL(digit_scan_kernel, dim3(P), dim3(256), 0, s, hist, base, n,
+                       trivial);
     for (int p = 0; p < P; ++p)
     {
-        const int src = p % 2, dst = 1 - src;
         timed_launch timer(ctx, values ? "radix_onesweep_pairs" : "radix_onesweep_keys");
-        onesweep_args a{ kb[src], kb[dst], vb[src], vb[dst], n, uint32_t(8 * p),
+        onesweep_args a{ kb[0], kb[1], vb[0], vb[1], n, uint32_t(p), trivial,
                          base + p * RADIX, st[p % 2],
                          p + 1 < P ? st[(p + 1) % 2] : nullptr, ctr + p };
         launch_onesweep<K, KIND>(shape, desc, values, wide, a, s);
     }
+    const uint32_t copy_grid =
+        std::max(1u, std::min<uint32_t>((n + 255) / 256, uint32_t(ctx.facts().cu_count) * 4));
+    hipLaunchKernelGGL((finish_copy_kernel<K>), dim3(copy_grid), dim3(256), 0, s, kb[0],
+                       kb[1], vb[0], vb[1], n, trivial, uint32_t(P));
     CLC_HIP(hipGetLastError());
 }
 

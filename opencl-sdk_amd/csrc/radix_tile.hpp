@@ -54,9 +54,13 @@ template <typename K, int KIND, bool DESC> struct key_digit
 // ---- one digit place --------------------------------------------------------
 
 //! One tile of one digit place. FULL tiles skip every bounds check; only
-//! the last tile of the array runs the guarded form.
+//! the last tile of the array runs the guarded form. With SPLIT, keys and
+//! values are reordered one after the other through the same LDS buffer
+//! (s_vals unused; each thread keeps its slots and destinations in
+//! registers), which halves the staging LDS of key-value tiles so larger
+//! tiles (longer contiguous digit runs in HBM) fit.
 template <typename K, typename S, bool VALUES, int BLOCK, int ITEMS, bool FULL,
-          typename DF>
+          bool SPLIT, typename DF>
 __device__ __forceinline__ void onesweep_tile(
     const K* __restrict__ keys_in, K* __restrict__ keys_out,
     const uint32_t* __restrict__ vals_in, uint32_t* __restrict__ vals_out,
@@ -144,7 +148,8 @@ __device__ __forceinline__ void onesweep_tile(
             const uint32_t d = dig(k[r]);
             const uint32_t slot = my_hist[d] + rank[r];
             s_keys[slot] = k[r];
-            if constexpr (VALUES) s_vals[slot] = v[r];
+            if constexpr (VALUES && !SPLIT) s_vals[slot] = v[r];
+            if constexpr (SPLIT) rank[r] = slot; // kept for the value pass
         }
     }
 
@@ -177,16 +182,46 @@ __device__ __forceinline__ void onesweep_tile(
     // Write the reordered tile: consecutive threads take consecutive slots,
     // which land in contiguous runs of one digit each.
     const uint32_t count = FULL ? TILE : n - tile_base;
-#pragma unroll
-    for (int j = 0; j < ITEMS; ++j)
+    if constexpr (SPLIT && VALUES)
     {
-        const uint32_t i = j * BLOCK + tid;
-        if (FULL || i < count)
+        uint32_t dst[ITEMS];
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j)
         {
-            const K key = s_keys[i];
-            const uint32_t dst = s_gbase[dig(key)] + i;
-            keys_out[dst] = key;
-            if constexpr (VALUES) vals_out[dst] = s_vals[i];
+            const uint32_t i = j * BLOCK + tid;
+            if (FULL || i < count)
+            {
+                const K key = s_keys[i];
+                dst[j] = s_gbase[dig(key)] + i;
+                keys_out[dst[j]] = key;
+            }
+        }
+        __syncthreads(); // keys out of the buffer: reuse it for the values
+        uint32_t* s_v = reinterpret_cast<uint32_t*>(s_keys);
+#pragma unroll
+        for (int r = 0; r < ITEMS; ++r)
+            if (FULL || wave_base + r * WAVE + lane < n) s_v[rank[r]] = v[r];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j)
+        {
+            const uint32_t i = j * BLOCK + tid;
+            if (FULL || i < count) vals_out[dst[j]] = s_v[i];
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j)
+        {
+            const uint32_t i = j * BLOCK + tid;
+            if (FULL || i < count)
+            {
+                const K key = s_keys[i];
+                const uint32_t dst = s_gbase[dig(key)] + i;
+                keys_out[dst] = key;
+                if constexpr (VALUES) vals_out[dst] = s_vals[i];
+            }
         }
     }
 }

@@ -234,11 +234,11 @@ __global__ __launch_bounds__(LARGE_BLOCK) void large_onesweep_kernel(
         const uint32_t* vi = VALUES ? vals_in + b : nullptr;
         uint32_t* vo = VALUES ? vals_out + b : nullptr;
         if (local * LARGE_TILE + LARGE_TILE <= m)
-            onesweep_tile<K, uint32_t, VALUES, LARGE_BLOCK, LARGE_ITEMS, true>(
+            onesweep_tile<K, uint32_t, VALUES, LARGE_BLOCK, LARGE_ITEMS, true, false>(
                 keys_in + b, keys_out + b, vi, vo, m, key_digit<K, KIND, DESC>{ 8u * pass }, dbase, st, local,
                 s_hist, s_keys, s_vals, s_gbase, s_tmp);
         else
-            onesweep_tile<K, uint32_t, VALUES, LARGE_BLOCK, LARGE_ITEMS, false>(
+            onesweep_tile<K, uint32_t, VALUES, LARGE_BLOCK, LARGE_ITEMS, false, false>(
                 keys_in + b, keys_out + b, vi, vo, m, key_digit<K, KIND, DESC>{ 8u * pass }, dbase, st, local,
                 s_hist, s_keys, s_vals, s_gbase, s_tmp);
         __syncthreads();
