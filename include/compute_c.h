@@ -248,6 +248,37 @@ clc_status clc_segmented_sort_graph_create(clc_context ctx, void* keys,
 clc_status clc_graph_launch(clc_graph graph);
 void clc_graph_release(clc_graph graph);
 
+/* ---- learned-bucket sort (no counterpart in the reference) ---------------- */
+
+/* Model quality of one clc_learned_sort call. */
+typedef struct clc_learned_stats
+{
+    int used_model; /* 0: the whole input went to the plain radix sort */
+    size_t num_buckets;
+    size_t sample_size;
+    /* estimated fraction of keys in buckets too large for LDS holding more
+     * than one distinct key (above 0.25 the model is not used) */
+    double sample_fit_error;
+    /* largest bucket / average bucket */
+    double max_bucket_overfill;
+    /* fraction of keys in buckets too large for LDS, finished by the
+     * onesweep fallback instead */
+    double fallback_fraction;
+} clc_learned_stats;
+
+/* Same result as clc_sort (stable, same key order, values permuted
+ * alongside), computed through a learned CDF model: a two-level RMI fitted
+ * to a strided 1% sample (sorted by the plain radix sort) maps keys
+ * monotonically to up to 65536 buckets, two stable passes keyed by the
+ * bucket id group the keys, and each bucket is finished in LDS; buckets too
+ * large for LDS, and inputs whose sample fits badly, go to the plain radix
+ * sort. n <= 2^31-1 (inputs below 2^16 or from 2^30 keys use the plain sort
+ * directly). Synchronizes once to check the fit, and once more to fill
+ * stats when stats != NULL. */
+clc_status clc_learned_sort(clc_context ctx, void* keys, clc_type key_type,
+                            void* values, size_t n, int descending,
+                            clc_learned_stats* stats);
+
 /* Whether clc_scan / clc_reduce support op/identity for type:
  * CLC_SUCCESS, CLC_ERROR_UNSUPPORTED (expression not recognised) or
  * CLC_ERROR_INVALID_ARGUMENT. Needs no device. */

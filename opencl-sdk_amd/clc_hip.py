@@ -56,6 +56,13 @@ class KernelTiming(ctypes.Structure):
                 ("max_ms", ctypes.c_double)]
 
 
+class LearnedStats(ctypes.Structure):
+    _fields_ = [("used_model", ctypes.c_int), ("num_buckets", ctypes.c_size_t),
+                ("sample_size", ctypes.c_size_t), ("sample_fit_error", ctypes.c_double),
+                ("max_bucket_overfill", ctypes.c_double),
+                ("fallback_fraction", ctypes.c_double)]
+
+
 # Every symbol include/compute_c.h declares, with (restype, argtypes).
 _vp, _sz, _i, _cp = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_char_p
 SIGNATURES = {
@@ -88,6 +95,7 @@ SIGNATURES = {
     "clc_segmented_sort_get_table": (_i, [_vp, _i, _i, ctypes.POINTER(ctypes.c_int)]),
     "clc_segmented_sort_graph_create": (_i, [_vp, _vp, _i, _vp, _sz, _vp, _sz, _i,
                                              ctypes.POINTER(_vp)]),
+    "clc_learned_sort": (_i, [_vp, _vp, _i, _vp, _sz, _i, ctypes.POINTER(LearnedStats)]),
     "clc_graph_launch": (_i, [_vp]),
     "clc_graph_release": (None, [_vp]),
 }
@@ -219,6 +227,20 @@ class Context:
             self.handle, ctypes.c_void_p(keys.data_ptr()), kt,
             None if values is None else ctypes.c_void_p(values.data_ptr()), n,
             ctypes.c_void_p(offsets.data_ptr()), ns, 1 if descending else 0))
+
+    def learned_sort(self, keys, values=None, descending=False, key_type=None, n=None,
+                     stats=True):
+        """clc_learned_sort; returns the model statistics as a dict (or None)."""
+        kt = dtype_code(keys.dtype) if key_type is None else key_type
+        n = keys.numel() if n is None else n
+        st = LearnedStats() if stats else None
+        check(lib().clc_learned_sort(
+            self.handle, ctypes.c_void_p(keys.data_ptr()), kt,
+            None if values is None else ctypes.c_void_p(values.data_ptr()), n,
+            1 if descending else 0, ctypes.byref(st) if st is not None else None))
+        if st is None:
+            return None
+        return {name: getattr(st, name) for name, _ in LearnedStats._fields_}
 
     def sort_tune(self, key_type, with_values=True) -> int:
         shape = ctypes.c_int(-1)

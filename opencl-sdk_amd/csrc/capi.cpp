@@ -13,6 +13,7 @@
 #include <string>
 
 #include "context.hpp"
+#include "learned_sort.hpp"
 #include "ops.hpp"
 #include "segmented_sort.hpp"
 
@@ -424,6 +425,27 @@ clc_status clc_graph_launch(clc_graph graph)
 }
 
 void clc_graph_release(clc_graph graph) { delete graph; }
+
+clc_status clc_learned_sort(clc_context ctx, void* keys, clc_type key_type,
+                            void* values, size_t n, int descending,
+                            clc_learned_stats* stats)
+{
+    return guarded([&] {
+        require(ctx != nullptr, "clc: null context");
+        clc::learned::stats st;
+        clc::detail::learned_sort(get(ctx), keys, key_type, values, n, descending != 0,
+                                  stats != nullptr ? &st : nullptr);
+        if (stats != nullptr)
+        {
+            stats->used_model = st.used_model;
+            stats->num_buckets = st.num_buckets;
+            stats->sample_size = st.sample_size;
+            stats->sample_fit_error = st.sample_fit_error;
+            stats->max_bucket_overfill = st.max_bucket_overfill;
+            stats->fallback_fraction = st.fallback_fraction;
+        }
+    });
+}
 
 clc_status clc_operator_supported(clc_type type, const char* op,
                                   const char* identity)

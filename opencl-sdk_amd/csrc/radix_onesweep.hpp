@@ -12,9 +12,11 @@ namespace radix {
 //! Tile shapes (workgroup size x keys per thread, split staging). The sort
 //! tuner picks one per key width and values flag; shape 0 is the default.
 constexpr int NOS = 8;
-constexpr int OS_BLOCK[NOS] = { 512, 256, 256, 1024, 512, 512, 512, 1024 };
-constexpr int OS_ITEMS[NOS] = { 16, 16, 24, 8, 16, 24, 32, 16 };
-constexpr bool OS_SPLIT[NOS] = { false, false, false, false, true, true, true, true };
+constexpr int OS_BLOCK[NOS] = { 1024, 512, 512, 1024, 1024, 768, 512, 1024 };
+constexpr int OS_ITEMS[NOS] = { 16, 16, 20, 16, 20, 16, 24, 12 };
+constexpr bool OS_SPLIT[NOS] = { true, false, true, false, true, true, true, true };
+//! LDS a workgroup may use on gfx950.
+constexpr std::size_t OS_LDS_LIMIT = 160 * 1024;
 
 inline uint32_t os_tile(int cfg) { return uint32_t(OS_BLOCK[cfg]) * OS_ITEMS[cfg]; }
 
@@ -26,6 +28,12 @@ inline std::size_t os_lds(int cfg, std::size_t kbytes, bool values)
     const std::size_t stage =
         OS_SPLIT[cfg] ? tile * kbytes : tile * kbytes + (values ? tile * 4 : 0);
     return std::size_t(OS_BLOCK[cfg] / 64) * 1024 + RADIX * 4 + stage + 64;
+}
+
+//! Whether shape cfg fits in LDS for this key width (shape 0 always does).
+inline bool os_feasible(int cfg, std::size_t kbytes, bool values)
+{
+    return cfg >= 0 && cfg < NOS && os_lds(cfg, kbytes, values) <= OS_LDS_LIMIT;
 }
 
 //! One digit place. The keys and values live in two buffers (a = the

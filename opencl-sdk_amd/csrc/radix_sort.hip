@@ -126,22 +126,23 @@ struct layout
 };
 
 //! Tile shape for a key width and values flag: the CLC_ONESWEEP_CFG
-//! environment variable, else the tuned shape, else shape 0.
+//! environment variable, else the tuned shape, else shape 0; a shape whose
+//! LDS does not fit this key width is never used.
 int pick_shape(context& ctx, std::size_t kbytes, bool values)
 {
     if (const char* e = std::getenv("CLC_ONESWEEP_CFG"))
     {
         const int c = std::atoi(e);
-        if (c >= 0 && c < NOS) return c;
+        if (os_feasible(c, kbytes, values)) return c;
     }
     auto& tuned = ctx.tuned();
     auto it = tuned.find(detail::sort_tuning_slot(kbytes, values));
-    if (it != tuned.end() && it->second.size() == 1 && it->second[0] >= 0
-        && it->second[0] < NOS)
+    if (it != tuned.end() && it->second.size() == 1
+        && os_feasible(it->second[0], kbytes, values))
         return it->second[0];
     std::vector<int> stored;
     if (load_tuning(ctx, detail::sort_tuning_name(kbytes, values), stored)
-        && stored.size() == 1 && stored[0] >= 0 && stored[0] < NOS)
+        && stored.size() == 1 && os_feasible(stored[0], kbytes, values))
     {
         tuned[detail::sort_tuning_slot(kbytes, values)] = stored;
         return stored[0];
@@ -303,6 +304,7 @@ int sort_tune(context& ctx, int key_type, bool with_values)
     float best_ms = 1e30f;
     for (int shape = 0; shape < NOS; ++shape)
     {
+        if (!os_feasible(shape, kb, with_values)) continue;
         float shape_ms = 1e30f;
         for (int rep = 0; rep < 4; ++rep)
         {

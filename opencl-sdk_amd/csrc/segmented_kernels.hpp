@@ -177,29 +177,83 @@ __global__ __launch_bounds__(256) void large_hist_kernel(
     }
 }
 
-//! Persistent: exclusive scan of each (large segment, digit place) count row.
+//! Persistent: exclusive scan of each (large segment, digit place) count
+//! row, and whether that place must run for that segment: a place where one
+//! digit holds every key of the segment is skipped (identity permutation).
 __global__ __launch_bounds__(256) void large_digit_scan_kernel(
     const uint32_t* __restrict__ hist, uint32_t* __restrict__ base,
-    const uint32_t* large_used, uint32_t P)
+    const uint32_t* large_used, uint32_t P, const uint64_t* __restrict__ off,
+    const uint32_t* __restrict__ large_list, uint32_t* __restrict__ exec)
 {
     __shared__ uint32_t tmp[4];
+    __shared__ uint32_t s_trivial;
     const uint32_t rows = *large_used * P;
     for (uint32_t r = blockIdx.x; r < rows; r += gridDim.x)
     {
+        if (threadIdx.x == 0) s_trivial = 0;
         const size_t i = size_t(r) * RADIX + threadIdx.x;
-        base[i] = block256_exclusive_add<256>(hist[i], tmp);
+        const uint32_t c = hist[i];
+        base[i] = block256_exclusive_add<256>(c, tmp); // ends with a barrier
+        const uint32_t s = large_list[r / P];
+        if (uint64_t(c) == off[s + 1] - off[s]) s_trivial = 1;
+        __syncthreads();
+        if (threadIdx.x == 0) exec[r] = s_trivial ? 0u : 1u;
+        __syncthreads();
+    }
+}
+
+//! Persistent over the large tiles: a segment that ran an odd number of
+//! digit places holds its result in the scratch buffers; copy it back.
+template <typename K, bool VALUES>
+__global__ __launch_bounds__(256) void large_copy_back_kernel(
+    K* __restrict__ keys, const K* __restrict__ keys_alt, uint32_t* __restrict__ vals,
+    const uint32_t* __restrict__ vals_alt, const uint64_t* __restrict__ off,
+    const uint32_t* __restrict__ large_list, const uint32_t* __restrict__ tile_start,
+    const uint32_t* large_used, const uint32_t* total_tiles,
+    const uint32_t* __restrict__ exec, uint32_t P, uint32_t* queue)
+{
+    __shared__ uint32_t s_item, s_j;
+    const uint32_t L = *large_used, T = *total_tiles;
+    for (;;)
+    {
+        if (threadIdx.x == 0)
+        {
+            const uint32_t t = atomicAdd(queue, 1u);
+            s_item = t;
+            if (t < T) s_j = find_segment(tile_start, L, t);
+        }
+        __syncthreads();
+        const uint32_t t = s_item, j = s_j;
+        __syncthreads();
+        if (t >= T) break;
+        uint32_t odd = 0;
+        for (uint32_t p = 0; p < P; ++p) odd ^= exec[j * P + p];
+        if (!odd) continue;
+        const uint32_t sidx = large_list[j];
+        const uint64_t b = off[sidx];
+        const uint64_t m = off[sidx + 1] - b;
+        const uint64_t lo = uint64_t(t - tile_start[j]) * LARGE_TILE;
+        const uint64_t hi = std::min<uint64_t>(lo + LARGE_TILE, m);
+        for (uint64_t i = lo + threadIdx.x; i < hi; i += 256)
+        {
+            keys[b + i] = keys_alt[b + i];
+            if constexpr (VALUES) vals[b + i] = vals_alt[b + i];
+        }
     }
 }
 
 //! Persistent onesweep over the tiles of every large segment, one digit
-//! place; look-back chains run within a segment.
+//! place; look-back chains run within a segment. Per segment, places whose
+//! digit is constant are skipped (exec[j][p] == 0) and each executed place
+//! reads what the previous executed place wrote (a = keys, b = scratch).
 template <typename K, int KIND, bool DESC, bool VALUES>
 __global__ __launch_bounds__(LARGE_BLOCK) void large_onesweep_kernel(
-    const K* keys_in, K* keys_out, const uint32_t* vals_in, uint32_t* vals_out,
+    K* keys_a, K* keys_b, uint32_t* vals_a, uint32_t* vals_b,
     const uint64_t* __restrict__ off, const uint32_t* __restrict__ large_list,
     const uint32_t* __restrict__ tile_start, const uint32_t* large_used,
     const uint32_t* total_tiles, uint32_t pass, const uint32_t* __restrict__ base,
-    uint32_t* status, uint32_t* status_next, uint32_t* queue)
+    const uint32_t* __restrict__ exec, uint32_t* status, uint32_t* status_next,
+    uint32_t* queue)
 {
     constexpr int P = sizeof(K);
     constexpr int W = LARGE_BLOCK / WAVE;
@@ -225,6 +279,17 @@ __global__ __launch_bounds__(LARGE_BLOCK) void large_onesweep_kernel(
         if (t >= T) break;
         const uint32_t j = s_j;
         if (status_next != nullptr && tid < RADIX) status_next[size_t(t) * RADIX + tid] = 0;
+        if (exec[j * P + pass] == 0)
+        {
+            __syncthreads(); // everyone has read s_item / s_j
+            continue;
+        }
+        uint32_t parity = 0;
+        for (uint32_t p = 0; p < pass; ++p) parity ^= exec[j * P + p];
+        const K* keys_in = parity ? keys_b : keys_a;
+        K* keys_out = parity ? keys_a : keys_b;
+        const uint32_t* vals_in = parity ? vals_b : vals_a;
+        uint32_t* vals_out = parity ? vals_a : vals_b;
         const uint32_t sidx = large_list[j];
         const uint64_t b = off[sidx];
         const uint32_t m = static_cast<uint32_t>(off[sidx + 1] - b);
@@ -233,14 +298,15 @@ __global__ __launch_bounds__(LARGE_BLOCK) void large_onesweep_kernel(
         uint32_t* st = status + size_t(tile_start[j]) * RADIX;
         const uint32_t* vi = VALUES ? vals_in + b : nullptr;
         uint32_t* vo = VALUES ? vals_out + b : nullptr;
+        const key_digit<K, KIND, DESC> dig{ 8u * pass };
         if (local * LARGE_TILE + LARGE_TILE <= m)
             onesweep_tile<K, uint32_t, VALUES, LARGE_BLOCK, LARGE_ITEMS, true, false>(
-                keys_in + b, keys_out + b, vi, vo, m, key_digit<K, KIND, DESC>{ 8u * pass }, dbase, st, local,
-                s_hist, s_keys, s_vals, s_gbase, s_tmp);
+                keys_in + b, keys_out + b, vi, vo, m, dig, dbase, st, local, s_hist, s_keys,
+                s_vals, s_gbase, s_tmp);
         else
             onesweep_tile<K, uint32_t, VALUES, LARGE_BLOCK, LARGE_ITEMS, false, false>(
-                keys_in + b, keys_out + b, vi, vo, m, key_digit<K, KIND, DESC>{ 8u * pass }, dbase, st, local,
-                s_hist, s_keys, s_vals, s_gbase, s_tmp);
+                keys_in + b, keys_out + b, vi, vo, m, dig, dbase, st, local, s_hist, s_keys,
+                s_vals, s_gbase, s_tmp);
         __syncthreads();
     }
 }
@@ -386,14 +452,17 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
     const std::size_t o_lists = Lt.take((NCFG + 1) * nseg * 4);
     const std::size_t o_tstart = Lt.take((L_max + 1) * 4);
     const std::size_t o_base = Lt.take(L_max * P * RADIX * 4);
+    const std::size_t o_exec = Lt.take(L_max * P * 4);
     const std::size_t o_stB = Lt.take(T_max * RADIX * 4);
     const std::size_t o_kalt = L_max ? Lt.take(num_items * sizeof(K)) : 0;
     const std::size_t o_valt = L_max && VALUES ? Lt.take(num_items * 4) : 0;
 
     auto hk = large_hist_kernel<K, KIND, DESC>;
     auto ok = large_onesweep_kernel<K, KIND, DESC, VALUES>;
+    auto ck = large_copy_back_kernel<K, VALUES>;
     const uint32_t hgrid = L_max ? cached_grid(ctx, hk, 256) : 0;
     const uint32_t ogrid = L_max ? cached_grid(ctx, ok, LARGE_BLOCK) : 0;
+    const uint32_t cpgrid = L_max ? cached_grid(ctx, ck, 256) : 0;
     if (prepare)
     {
         for (int c = 0; c < NCFG; ++c)
@@ -408,13 +477,15 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
     uint32_t* ctr = reinterpret_cast<uint32_t*>(base_ptr + o_ctr);
     // counters: [0, NCFG] list counts, [8, 8 + NCFG) LDS-class queues,
     // 16 histogram queue, 17 tiles in use, 18 large segments in use,
-    // [24, 24 + P) onesweep queues
+    // 19 copy-back queue, [24, 24 + P) onesweep queues
     uint32_t* counts = ctr;
     uint32_t* lds_q = ctr + 8;
     uint32_t* hist_q = ctr + 16;
     uint32_t* total_tiles = ctr + 17;
     uint32_t* large_used = ctr + 18;
+    uint32_t* copy_q = ctr + 19;
     uint32_t* pass_q = ctr + 24;
+    uint32_t* exec = reinterpret_cast<uint32_t*>(base_ptr + o_exec);
     uint32_t* lists = reinterpret_cast<uint32_t*>(base_ptr + o_lists);
     uint32_t* tstart = reinterpret_cast<uint32_t*>(base_ptr + o_tstart);
     uint32_t* hist = reinterpret_cast<uint32_t*>(base_ptr + o_hist);
@@ -444,19 +515,21 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
                            large_used, total_tiles, hist, hist_q);
         const uint32_t sgrid = static_cast<uint32_t>(std::min<std::size_t>(L_max * P, 4096));
         hipLaunchKernelGGL(large_digit_scan_kernel, dim3(sgrid), dim3(256), 0, s, hist,
-                           dbase, large_used, uint32_t(P));
+                           dbase, large_used, uint32_t(P), off, llist, exec);
         K* kb[2] = { keys, reinterpret_cast<K*>(base_ptr + o_kalt) };
         uint32_t* vb[2] = { vals, VALUES ? reinterpret_cast<uint32_t*>(base_ptr + o_valt)
                                          : nullptr };
         for (int p = 0; p < P; ++p)
         {
-            const int src = p % 2, dst = 1 - src;
             timed_launch timer(ctx, "segmented_onesweep");
-            hipLaunchKernelGGL(ok, dim3(ogrid), dim3(LARGE_BLOCK), 0, s, kb[src], kb[dst],
-                               vb[src], vb[dst], off, llist, tstart, large_used,
-                               total_tiles, uint32_t(p), dbase, st[p % 2],
+            hipLaunchKernelGGL(ok, dim3(ogrid), dim3(LARGE_BLOCK), 0, s, kb[0], kb[1],
+                               vb[0], vb[1], off, llist, tstart, large_used, total_tiles,
+                               uint32_t(p), dbase, exec, st[p % 2],
                                p + 1 < P ? st[(p + 1) % 2] : nullptr, pass_q + p);
         }
+        hipLaunchKernelGGL(ck, dim3(cpgrid), dim3(256), 0, s, kb[0], kb[1], vb[0], vb[1],
+                           off, llist, tstart, large_used, total_tiles, exec, uint32_t(P),
+                           copy_q);
     }
     CLC_HIP(hipGetLastError());
     return Lt.bytes;

@@ -168,6 +168,32 @@ def lognormal_segments(nseg: int, mu: float, sigma: float, lo: int, hi: int,
     return off
 
 
+def zipf_keys(n: int, s: float, hi: int, seed: int) -> np.ndarray:
+    """Zipf(s) ranks truncated to [1, hi] by rejection (heavy duplicates)."""
+    rng = np.random.default_rng(seed)
+    out = rng.zipf(s, n).astype(np.uint64)
+    bad = out > hi
+    while bad.any():
+        out[bad] = rng.zipf(s, int(bad.sum())).astype(np.uint64)
+        bad = out > hi
+    return out
+
+
+def lognormal_keys(n: int, mu: float, sigma: float, seed: int) -> np.ndarray:
+    """round(lognormal(mu, sigma)) as uint64."""
+    rng = np.random.default_rng(seed)
+    return np.rint(rng.lognormal(mu, sigma, n)).astype(np.uint64)
+
+
+def gaussian_mixture_keys(n: int, k: int, span: float, sd: float, seed: int) -> np.ndarray:
+    """k Gaussians with means uniform in [0, span) and standard deviation sd,
+    clamped to [0, 2^64 - 1], as uint64."""
+    rng = np.random.default_rng(seed)
+    means = rng.uniform(0.0, span, k)
+    x = means[rng.integers(0, k, n)] + rng.normal(0.0, sd, n)
+    return np.clip(x, 0.0, 2.0**64 - 2.0**12).astype(np.uint64)
+
+
 def scan(x: np.ndarray, op: str = "plus", identity=0, exclusive: bool = False) -> np.ndarray:
     """Sequential left-to-right scan (Algorithm.cpp:76-127, :280-318)."""
     x = np.ascontiguousarray(x)
