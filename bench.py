@@ -1,20 +1,35 @@
 #!/usr/bin/env python3
-"""Benchmark of the hot path on one MI355X: stable key-value radix sort of
-2^28 uint32 keys (uniform over 32 bits) with uint32 values 0..n-1, through
-the C ABI (clc_sort).
+"""Benchmark of the hot path on one MI355X, through the C ABI.
 
-    python bench.py [--steps K] [--warmup W] [--log2n 28] [--no-cpu]
+Main line: stable key-value radix sort (clc_sort) of 2^28 uint32 keys
+uniform over 32 bits with uint32 values 0..n-1, ascending.
+
+    python bench.py [--steps K] [--warmup W] [--log2n 28] [--key u32|u64]
+                    [--legs u64,learned,segmented | none] [--no-cpu]
 
 A step is one clc_sort call on inputs already resident in HBM. Before each
 step the pristine keys/values are restored by a device-to-device copy that
 lies outside the step's hipEvent pair, so `value` counts only the sort.
-Prints one JSON line (metric, value in GKeys/s, roofline of the dominant
-kernel, CPU baseline).
+Prints one JSON line: metric, value in GKeys/s, the roofline of the dominant
+kernel (onesweep pass), the CPU baseline, and under "legs" the other
+workloads of the same path, each timed the same way:
+
+  u64        2^27 uint64 keys uniform over [0, 2^48) + uint32 values;
+  learned    clc_learned_sort against clc_sort on 2^28 uint64 keys + uint32
+             values from three skewed distributions (Zipf s=1.2 over
+             [1, 2^40], round(lognormal(20, 2)), 8 Gaussians with means in
+             [0, 2^60) and sd 2^40); outputs compared bit for bit;
+  segmented  65,536 segments with lengths round(exp(N(ln 512, 1.5)))
+             clamped to [1, 2^20], uint32 keys + segment-local indices:
+             one captured clc_graph launch against one clc_sort call per
+             segment; outputs compared bit for bit.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
+import math
 import os
 import statistics
 import sys
@@ -25,6 +40,10 @@ sys.path.insert(0, os.path.join(ROOT, "opencl-sdk_amd"))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 METRIC = "GKeys/s + % HBM roofline, 2^28 u32 pairs; learned-bucket vs plain radix"
+
+
+def say(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
 
 
 def cpu_baseline(log2n: int, reps: int) -> dict:
@@ -52,34 +71,27 @@ def cpu_baseline(log2n: int, reps: int) -> dict:
                       f"({threads} threads, {best:.2f} s each)"}
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--log2n", type=int, default=None,
-                    help="log2 of the key count (default 28 for u32, 27 for u64)")
-    ap.add_argument("--key", choices=["u32", "u64"], default="u32",
-                    help="u32: full 32-bit keys; u64: keys uniform over [0, 2^48)")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-log2n", type=int, default=26)
-    ap.add_argument("--cpu-reps", type=int, default=2)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this benchmark runs on exactly one GPU (asked for {args.gpus}); "
-              "the sort has no multi-GPU path in this build", file=sys.stderr)
-        return 2
+def event_pairs(torch, k):
+    return ([torch.cuda.Event(enable_timing=True) for _ in range(k)],
+            [torch.cuda.Event(enable_timing=True) for _ in range(k)])
 
-    import torch
-    import clc_hip
 
-    wide = args.key == "u64"
-    log2n = args.log2n if args.log2n is not None else (27 if wide else 28)
+def check_sorted_pairs(torch, keys0, keys, vals, wide):
+    """Ascending keys, values a permutation consistent with the keys, ties
+    in index order (stable)."""
+    k = keys if wide else keys.to(torch.int64) & 0xFFFFFFFF
+    ok = bool((k[1:] >= k[:-1]).all())
+    ok = ok and bool((keys0[vals.long()] == keys).all())
+    ties = k[1:] == k[:-1]
+    return ok and bool((vals[1:][ties] > vals[:-1][ties]).all())
+
+
+def plain_sort(torch, clc_hip, ctx, wide, log2n, steps, warmup, traffic_dir=None):
+    """Times clc_sort on uniform keys; returns the measurements."""
     n = 1 << log2n
     kbytes = 8 if wide else 4
     passes = 8 if wide else 4
     executed = 6 if wide else 4  # u64 keys below 2^48: the top two bytes are constant
-    torch.cuda.set_device(0)
     g = torch.Generator(device="cuda")
     g.manual_seed(42)
     if wide:
@@ -90,28 +102,24 @@ def main() -> int:
     vals0 = torch.arange(n, dtype=torch.int32, device="cuda")
     keys = torch.empty_like(keys0)
     vals = torch.empty_like(vals0)
-    ctx = clc_hip.Context()
+    kt = clc_hip.U64 if wide else clc_hip.U32
 
-    def step():
-        ctx.sort(keys, vals, key_type=clc_hip.U64 if wide else clc_hip.U32)
-
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         keys.copy_(keys0)
         vals.copy_(vals0)
-        step()
+        ctx.sort(keys, vals, key_type=kt)
     torch.cuda.synchronize()
 
     ctx.reset_timings()
     ctx.set_profiling(True)
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    starts, ends = event_pairs(torch, steps)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for i in range(args.steps):
+    for i in range(steps):
         keys.copy_(keys0)
         vals.copy_(vals0)
         starts[i].record()
-        step()
+        ctx.sort(keys, vals, key_type=kt)
         ends[i].record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -119,14 +127,7 @@ def main() -> int:
     step_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
     ms = statistics.fmean(step_ms)
     timings = ctx.kernel_timings()
-
-    # Output check on the last step: ascending keys, values a permutation
-    # consistent with the keys, ties in index order (stable).
-    k = keys if wide else keys.to(torch.int64) & 0xFFFFFFFF
-    ok = bool((k[1:] >= k[:-1]).all())
-    ok = ok and bool((keys0[vals.long()] == keys).all())
-    ties = k[1:] == k[:-1]
-    ok = ok and bool((vals[1:][ties] > vals[:-1][ties]).all())
+    ok = check_sorted_pairs(torch, keys0, keys, vals, wide)
 
     one = timings.get("radix_onesweep_pairs")
     roofline = None
@@ -138,11 +139,12 @@ def main() -> int:
         bytes_per_launch = n * 2 * (kbytes + 4)  # read + write of key and value
         achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
         traffic = None
-        tpath = os.path.join(ROOT, "profiles",
-                             f"onesweep_pairs_{args.key}_2p{log2n}_traffic.json")
-        if os.path.exists(tpath):
-            with open(tpath) as f:
-                traffic = json.load(f).get("hbm_bytes_per_launch")
+        if traffic_dir is not None:
+            tpath = os.path.join(traffic_dir, f"onesweep_pairs_{'u64' if wide else 'u32'}"
+                                              f"_2p{log2n}_traffic.json")
+            if os.path.exists(tpath):
+                with open(tpath) as f:
+                    traffic = json.load(f).get("hbm_bytes_per_launch")
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": traffic, "kernel": "radix_onesweep_pairs",
@@ -150,15 +152,280 @@ def main() -> int:
                     "algorithmic_bytes_per_launch": bytes_per_launch}
     sort_bytes = n * (kbytes + passes * 2 * (kbytes + 4))  # k + P*2(k+v)
     executed_bytes = n * (kbytes + executed * 2 * (kbytes + 4))
-    value = n / (ms / 1e3) / 1e9
+    del keys0, vals0, keys, vals
+    return {
+        "n": n, "kbytes": kbytes, "passes": passes, "executed": executed,
+        "ms": ms, "gkeys": n / (ms / 1e3) / 1e9, "ok": ok, "roofline": roofline,
+        "sort_roofline_frac": sort_bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+        "executed_pass_roofline_frac": executed_bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+        "step_ms": step_ms, "wall": wall,
+        "kernels": {k2: {"launches": v2["launches"],
+                         "avg_ms": round(v2["total_ms"] / max(v2["launches"], 1), 4)}
+                    for k2, v2 in timings.items()},
+    }
+
+
+# ---- skewed u64 keys, generated on the GPU ----------------------------------
+
+def zipf_keys(torch, n, s, hi, gen):
+    """Zipf(s) ranks over [1, hi] by rejection-inversion (Hoermann and
+    Derflinger, 1996): invert H(x) = (x^(1-s) - 1) / (1-s), the integral of
+    x^-s, and accept k = round(x) if k - x <= c or u >= H(k + 1/2) - k^-s."""
+    a = 1.0 - s
+
+    def H(x):
+        return (x ** a - 1.0) / a
+
+    def Hinv(y):
+        return (1.0 + y * a) ** (1.0 / a)
+
+    h_lo = H(1.5) - 1.0
+    h_hi = H(hi + 0.5)
+    c = 2.0 - Hinv(H(2.5) - 2.0 ** -s)
+    out = torch.empty(n, dtype=torch.int64, device="cuda")
+    todo = torch.arange(n, device="cuda")
+    while todo.numel():
+        u = h_hi + torch.rand(todo.numel(), dtype=torch.float64, device="cuda",
+                              generator=gen) * (h_lo - h_hi)
+        x = torch.pow(1.0 + u * a, 1.0 / a)
+        k = torch.floor(x + 0.5).clamp_(1.0, float(hi))
+        ok = (k - x <= c) | (u >= (torch.pow(k + 0.5, a) - 1.0) / a - torch.pow(k, -s))
+        out[todo[ok]] = k[ok].to(torch.int64)
+        todo = todo[~ok]
+        del u, x, k, ok
+    return out
+
+
+def lognormal_keys(torch, n, mu, sigma, gen):
+    x = torch.randn(n, dtype=torch.float64, device="cuda", generator=gen)
+    return torch.exp(x.mul_(sigma).add_(mu)).round_().to(torch.int64)
+
+
+def gaussian_mixture_keys(torch, n, k, span, sd, gen):
+    means = torch.rand(k, dtype=torch.float64, device="cuda", generator=gen) * span
+    comp = torch.randint(0, k, (n,), device="cuda", generator=gen)
+    x = torch.randn(n, dtype=torch.float64, device="cuda", generator=gen).mul_(sd)
+    x += means[comp]
+    del comp
+    # clamp to [0, 2^64 - 1]; every mean + 6 sd stays far below 2^63
+    return x.clamp_(min=0.0).to(torch.int64)
+
+
+LEARNED_DISTS = {
+    "zipf_s1.2_2^40": lambda torch, n, g: zipf_keys(torch, n, 1.2, 1 << 40, g),
+    "lognormal_20_2": lambda torch, n, g: lognormal_keys(torch, n, 20.0, 2.0, g),
+    "gauss_mix_8": lambda torch, n, g: gaussian_mixture_keys(torch, n, 8, 2.0 ** 60,
+                                                             2.0 ** 40, g),
+}
+
+
+def learned_leg(torch, clc_hip, ctx, log2n, steps, warmup):
+    n = 1 << log2n
+    kb, vb = 8, 4
+    out = {"n": n, "key": "u64", "value": "u32 index",
+           "algorithmic_bytes_per_pair": {"learned": kb + 4 * (kb + vb),
+                                          "plain": kb + 8 * 2 * (kb + vb)}}
+    g = torch.Generator(device="cuda")
+    for i, (name, gen) in enumerate(LEARNED_DISTS.items()):
+        say("learned leg", name, f"2^{log2n}")
+        g.manual_seed(1000 + i)
+        keys0 = gen(torch, n, g)
+        vals0 = torch.arange(n, dtype=torch.int32, device="cuda")
+        kl, vl = torch.empty_like(keys0), torch.empty_like(vals0)
+        kp, vp = torch.empty_like(keys0), torch.empty_like(vals0)
+
+        def learned():
+            kl.copy_(keys0)
+            vl.copy_(vals0)
+            e0.record()
+            ctx.learned_sort(kl, vl, key_type=clc_hip.U64, stats=False)
+            e1.record()
+
+        def plain():
+            kp.copy_(keys0)
+            vp.copy_(vals0)
+            e0.record()
+            ctx.sort(kp, vp, key_type=clc_hip.U64)
+            e1.record()
+
+        res = {}
+        for label, fn in (("learned", learned), ("plain", plain)):
+            for _ in range(warmup):
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                fn()
+            times = []
+            for _ in range(steps):
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                fn()
+                torch.cuda.synchronize()
+                times.append(e0.elapsed_time(e1))
+            res[label] = statistics.fmean(times)
+        same = bool(torch.equal(kl, kp)) and bool(torch.equal(vl, vp))
+        # model statistics from one more call (reading them back adds a sync)
+        kl.copy_(keys0)
+        vl.copy_(vals0)
+        st = ctx.learned_sort(kl, vl, key_type=clc_hip.U64, stats=True)
+        torch.cuda.synchronize()
+        same = same and bool(torch.equal(kl, kp)) and bool(torch.equal(vl, vp))
+        distinct_top = int(torch.unique(keys0[: 1 << 20]).numel())
+        out[name] = {
+            "learned_ms": round(res["learned"], 4), "plain_ms": round(res["plain"], 4),
+            "learned_gkeys": round(n / res["learned"] / 1e6, 3),
+            "plain_gkeys": round(n / res["plain"] / 1e6, 3),
+            "speedup": round(res["plain"] / res["learned"], 3),
+            "learned_roofline_frac": round((kb + 4 * (kb + vb)) * n / (res["learned"] / 1e3)
+                                           / 1e9 / HBM_PEAK_GBS, 4),
+            "bit_identical_to_plain": same,
+            "used_model": st["used_model"], "num_buckets": st["num_buckets"],
+            "sample_size": st["sample_size"],
+            "sample_fit_error": round(st["sample_fit_error"], 5),
+            "max_bucket_overfill": round(st["max_bucket_overfill"], 3),
+            "fallback_fraction": round(st["fallback_fraction"], 5),
+            "distinct_keys_in_first_2^20": distinct_top,
+        }
+        say(name, out[name])
+        del keys0, vals0, kl, vl, kp, vp
+        torch.cuda.empty_cache()
+    return out
+
+
+def segmented_leg(torch, clc_hip, ctx, steps, warmup):
+    import numpy as np
+    nseg = 65536
+    rng = np.random.default_rng(2024)
+    sizes = np.clip(np.rint(np.exp(rng.normal(math.log(512.0), 1.5, nseg))), 1,
+                    1 << 20).astype(np.int64)
+    off = np.zeros(nseg + 1, dtype=np.int64)
+    np.cumsum(sizes, out=off[1:])
+    total = int(off[-1])
+    say("segmented leg", nseg, "segments,", total, "pairs")
+    g = torch.Generator(device="cuda")
+    g.manual_seed(77)
+    keys0 = torch.randint(-(1 << 31), (1 << 31) - 1, (total,), dtype=torch.int32,
+                          device="cuda", generator=g)
+    off_d = torch.from_numpy(off).cuda()
+    seg_of = torch.repeat_interleave(torch.arange(nseg, device="cuda"),
+                                     torch.from_numpy(sizes).cuda())
+    vals0 = (torch.arange(total, device="cuda") - off_d[seg_of]).to(torch.int32)
+    del seg_of
+    keys = torch.empty_like(keys0)
+    vals = torch.empty_like(vals0)
+
+    keys.copy_(keys0)
+    vals.copy_(vals0)
+    graph = ctx.segmented_sort_graph(keys, vals, off_d, key_type=clc_hip.U32)
+
+    def timed(fn, reps):
+        t = []
+        for _ in range(reps):
+            keys.copy_(keys0)
+            vals.copy_(vals0)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            t.append(e0.elapsed_time(e1))
+        return t
+
+    timed(graph.launch, warmup)
+    graph_ms = statistics.fmean(timed(graph.launch, steps))
+    kg, vg = keys.clone(), vals.clone()
+
+    def direct():
+        ctx.segmented_sort(keys, vals, off_d, key_type=clc_hip.U32)
+
+    timed(direct, warmup)
+    direct_ms = statistics.fmean(timed(direct, steps))
+    same_direct = bool(torch.equal(keys, kg)) and bool(torch.equal(vals, vg))
+
+    # One clc_sort call per segment, issued from a host loop over raw pointers.
+    L = clc_hip.lib()
+    kbase, vbase = keys.data_ptr(), vals.data_ptr()
+    handle = ctx.handle
+    segs = [(int(o) * 4, int(s)) for o, s in zip(off[:-1], sizes)]
+
+    def per_segment(limit=None):
+        for o, s in segs[:limit]:
+            st = L.clc_sort(handle, ctypes.c_void_p(kbase + o), clc_hip.U32,
+                            ctypes.c_void_p(vbase + o), s, 0)
+            if st != 0:
+                clc_hip.check(st)
+
+    timed(lambda: per_segment(4096), 1)
+    t0 = time.perf_counter()
+    per_seg_ms = timed(per_segment, 1)[0]
+    per_seg_wall = time.perf_counter() - t0
+    same_per_seg = bool(torch.equal(keys, kg)) and bool(torch.equal(vals, vg))
+    graph.close()
+    algo = total * 2 * (4 + 4)
+    out = {
+        "segments": nseg, "pairs": total, "min_len": int(sizes.min()),
+        "median_len": float(np.median(sizes)), "max_len": int(sizes.max()),
+        "graph_ms": round(graph_ms, 4), "direct_call_ms": round(direct_ms, 4),
+        "per_segment_launches_ms": round(per_seg_ms, 2),
+        "per_segment_wall_s": round(per_seg_wall, 3),
+        "speedup_vs_per_segment": round(per_seg_ms / graph_ms, 1),
+        "graph_gkeys": round(total / graph_ms / 1e6, 3),
+        "graph_lds_roofline_frac": round(algo / (graph_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        "bit_identical": same_direct and same_per_seg,
+    }
+    say("segmented", out)
+    del keys0, vals0, keys, vals, kg, vg, off_d
+    torch.cuda.empty_cache()
+    return out
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--log2n", type=int, default=None,
+                    help="log2 of the key count (default 28 for u32, 27 for u64)")
+    ap.add_argument("--key", choices=["u32", "u64"], default="u32",
+                    help="u32: full 32-bit keys; u64: keys uniform over [0, 2^48)")
+    ap.add_argument("--legs", default="u64,learned,segmented",
+                    help="extra workloads to report: comma list of u64, learned, "
+                         "segmented, or 'none'")
+    ap.add_argument("--learned-log2n", type=int, default=28)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-log2n", type=int, default=28)
+    ap.add_argument("--cpu-reps", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this benchmark runs on exactly one GPU (asked for {args.gpus}); "
+              "the sort has no multi-GPU path in this build", file=sys.stderr)
+        return 2
+    legs = [] if args.legs == "none" else [x for x in args.legs.split(",") if x]
+    for x in legs:
+        if x not in ("u64", "learned", "segmented"):
+            ap.error(f"unknown leg {x}")
+
+    import torch
+    import clc_hip
+
+    wide = args.key == "u64"
+    log2n = args.log2n if args.log2n is not None else (27 if wide else 28)
+    torch.cuda.set_device(0)
+    ctx = clc_hip.Context()
+
+    say("main", args.key, f"2^{log2n}")
+    m = plain_sort(torch, clc_hip, ctx, wide, log2n, args.steps, args.warmup,
+                   traffic_dir=os.path.join(ROOT, "profiles"))
+    ok = m["ok"]
+    step_ms = m["step_ms"]
     line = {
         "metric": METRIC,
-        "value": round(value, 3),
+        "value": round(m["gkeys"], 3),
         "unit": "GKeys/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms, 4),
+        "ms_per_step": round(m["ms"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -168,23 +435,46 @@ def main() -> int:
                                 "over [0, 2^48) + uint32 index values, ascending") if wide else
                                (f"stable key-value radix sort, 2^{log2n} uint32 keys "
                                 "uniform over 32 bits + uint32 index values, ascending"),
-                   "n": n, "key_bytes": kbytes, "value_bytes": 4, "passes": passes,
-                   "executed_passes": executed, "parallelism": "single GPU"},
-        "roofline": roofline,
+                   "n": m["n"], "key_bytes": m["kbytes"], "value_bytes": 4,
+                   "passes": m["passes"], "executed_passes": m["executed"],
+                   "parallelism": "single GPU"},
+        "roofline": m["roofline"],
         "cpu_baseline": None,
-        "extra": {"sort_roofline_frac": round(sort_bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
-                  "executed_pass_roofline_frac":
-                      round(executed_bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        "extra": {"sort_roofline_frac": round(m["sort_roofline_frac"], 4),
+                  "executed_pass_roofline_frac": round(m["executed_pass_roofline_frac"], 4),
                   "step_ms_min": round(min(step_ms), 4),
                   "step_ms_median": round(statistics.median(step_ms), 4),
                   "step_ms_max": round(max(step_ms), 4),
-                  "wall_s_timed_loop_incl_restore_copies": round(wall, 3),
+                  "wall_s_timed_loop_incl_restore_copies": round(m["wall"], 3),
                   "output_check": ok,
-                  "kernels": {k2: {"launches": v2["launches"],
-                                   "avg_ms": round(v2["total_ms"] / max(v2["launches"], 1), 4)}
-                              for k2, v2 in timings.items()}},
+                  "kernels": m["kernels"]},
+        "legs": {},
     }
+    torch.cuda.empty_cache()
+    if "u64" in legs and not wide:
+        say("leg u64 2^27")
+        u = plain_sort(torch, clc_hip, ctx, True, 27, args.steps, args.warmup,
+                       traffic_dir=os.path.join(ROOT, "profiles"))
+        ok = ok and u["ok"]
+        line["legs"]["u64_2^27_uniform48"] = {
+            "gkeys": round(u["gkeys"], 3), "ms": round(u["ms"], 4), "output_check": u["ok"],
+            "executed_passes": u["executed"], "roofline": u["roofline"],
+            "sort_roofline_frac": round(u["sort_roofline_frac"], 4),
+            "executed_pass_roofline_frac": round(u["executed_pass_roofline_frac"], 4)}
+        torch.cuda.empty_cache()
+    if "learned" in legs:
+        lr = learned_leg(torch, clc_hip, ctx, args.learned_log2n, max(3, args.steps // 4),
+                         1)
+        ok = ok and all(v["bit_identical_to_plain"] for k, v in lr.items()
+                        if isinstance(v, dict) and "bit_identical_to_plain" in v)
+        line["legs"]["learned"] = lr
+    if "segmented" in legs:
+        sg = segmented_leg(torch, clc_hip, ctx, max(3, args.steps // 2), 2)
+        ok = ok and sg["bit_identical"]
+        line["legs"]["segmented"] = sg
+    line["extra"]["output_check"] = ok
     if not args.no_cpu:
+        say("cpu baseline")
         line["cpu_baseline"] = cpu_baseline(args.cpu_log2n, args.cpu_reps)
     print(json.dumps(line), flush=True)
     return 0 if ok else 1

@@ -1,10 +1,11 @@
 """Builds libclc_hip.so (the C-ABI library) for gfx950, in-tree.
 
-    python opencl-sdk_amd/build.py [--jobs N]
+    python opencl-sdk_amd/build.py [--jobs N] [--incremental]
 
 Every source under csrc/ is compiled by hipcc for gfx950 into _build/ and
 linked into libclc_hip.so beside this file. hipcc cross-compiles without a
-GPU.
+GPU. --incremental skips sources whose object is newer than the source and
+every header (any header change recompiles everything).
 """
 from __future__ import annotations
 
@@ -44,12 +45,30 @@ def _compile(src: str, extra: list[str]) -> str:
     return obj
 
 
-def build(jobs: int = 8, verbose: bool = False) -> str:
+def _fresh(src: str, newest_header: float) -> bool:
+    obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+    if not os.path.exists(obj):
+        return False
+    t = os.path.getmtime(obj)
+    return t > os.path.getmtime(src) and t > newest_header
+
+
+def build(jobs: int = 8, verbose: bool = False, incremental: bool = False) -> str:
+    """Compiles every source (or, with incremental, those older than their
+    object or than any header under csrc/) and links the library."""
     os.makedirs(BUILD, exist_ok=True)
     extra: list[str] = []
     srcs = sources()
+    headers = glob.glob(os.path.join(CSRC, "*.hpp")) + glob.glob(os.path.join(CSRC, "*.h"))
+    newest = max((os.path.getmtime(h) for h in headers), default=0.0)
+
+    def one(s):
+        if incremental and _fresh(s, newest):
+            return os.path.join(BUILD, os.path.basename(s) + ".o")
+        return _compile(s, extra)
+
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
-        objs = list(ex.map(lambda s: _compile(s, extra), srcs))
+        objs = list(ex.map(one, srcs))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -64,8 +83,10 @@ def build(jobs: int = 8, verbose: bool = False) -> str:
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=8)
+    ap.add_argument("--incremental", action="store_true",
+                    help="recompile only sources older than their object or any header")
     args = ap.parse_args()
-    build(jobs=args.jobs, verbose=True)
+    build(jobs=args.jobs, verbose=True, incremental=args.incremental)
 
 
 if __name__ == "__main__":

@@ -250,20 +250,29 @@ __global__ __launch_bounds__(256) void id_histogram_kernel(const K* __restrict__
                                                            uint32_t n, const rmi_model* model,
                                                            uint32_t* __restrict__ hist)
 {
+    // C interleaved copies per bin (copy = lane mod C): skewed keys put many
+    // lanes on one bucket, which would serialise 64 deep on a single copy.
+    constexpr uint32_t C = 8;
     __shared__ rmi_model sm;
-    __shared__ uint32_t s_h[2 * RADIX];
+    __shared__ uint32_t s_h[2 * RADIX * C];
     load_model(&sm, model);
-    for (uint32_t i = threadIdx.x; i < 2 * RADIX; i += 256) s_h[i] = 0;
+    for (uint32_t i = threadIdx.x; i < 2 * RADIX * C; i += 256) s_h[i] = 0;
     __syncthreads();
+    const uint32_t copy = threadIdx.x % C;
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
     {
         const uint32_t b = bucket_of(sm, static_cast<uint64_t>(order_bits<K, KIND, DESC>(keys[i])));
-        atomicAdd(&s_h[b & 0xFFu], 1u);
-        atomicAdd(&s_h[RADIX + (b >> 8)], 1u);
+        atomicAdd(&s_h[(b & 0xFFu) * C + copy], 1u);
+        atomicAdd(&s_h[(RADIX + (b >> 8)) * C + copy], 1u);
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < 2 * RADIX; i += 256)
-        if (s_h[i] != 0) atomicAdd(&hist[i], s_h[i]);
+    {
+        uint32_t c = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < C; ++j) c += s_h[i * C + j];
+        if (c != 0) atomicAdd(&hist[i], c);
+    }
 }
 
 __global__ __launch_bounds__(256) void id_scan_kernel(const uint32_t* __restrict__ hist,

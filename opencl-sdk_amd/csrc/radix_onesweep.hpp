@@ -9,12 +9,16 @@
 namespace clc {
 namespace radix {
 
-//! Tile shapes (workgroup size x keys per thread, split staging). The sort
-//! tuner picks one per key width and values flag; shape 0 is the default.
+//! Tile shapes: workgroup size x keys per thread, split staging, and the
+//! waves per SIMD the register allocation must leave room for (4 with
+//! 512-thread groups = two groups per CU when their LDS fits, so one ranks
+//! while the other waits on its look-back or on HBM). The sort tuner picks
+//! one per key width and values flag; os_default() is used untuned.
 constexpr int NOS = 8;
-constexpr int OS_BLOCK[NOS] = { 1024, 512, 512, 1024, 1024, 768, 512, 1024 };
-constexpr int OS_ITEMS[NOS] = { 16, 16, 20, 16, 20, 16, 24, 12 };
-constexpr bool OS_SPLIT[NOS] = { true, false, true, false, true, true, true, true };
+constexpr int OS_BLOCK[NOS] = { 1024, 512, 512, 1024, 512, 512, 1024, 512 };
+constexpr int OS_ITEMS[NOS] = { 16, 16, 20, 16, 12, 16, 20, 12 };
+constexpr bool OS_SPLIT[NOS] = { true, false, true, false, false, true, true, true };
+constexpr int OS_WAVES[NOS] = { 4, 4, 4, 4, 4, 4, 4, 6 };
 //! LDS a workgroup may use on gfx950.
 constexpr std::size_t OS_LDS_LIMIT = 160 * 1024;
 
@@ -34,6 +38,15 @@ inline std::size_t os_lds(int cfg, std::size_t kbytes, bool values)
 inline bool os_feasible(int cfg, std::size_t kbytes, bool values)
 {
     return cfg >= 0 && cfg < NOS && os_lds(cfg, kbytes, values) <= OS_LDS_LIMIT;
+}
+
+//! Untuned shape: the largest tiles that fit one workgroup per CU. Measured
+//! on MI355X, 2^28 u32 pairs: shape 6 (1024 x 20, split) 58.1 GKeys/s,
+//! shape 3 56.3, shape 0 53.4, every two-groups-per-CU shape 42.8-51.8;
+//! 2^27 u64 pairs: shape 0 (1024 x 16, split) 27.3, the rest 19.2-24.1.
+inline int os_default(std::size_t kbytes, bool values)
+{
+    return kbytes == 4 && os_feasible(6, kbytes, values) ? 6 : 0;
 }
 
 //! One digit place. The keys and values live in two buffers (a = the
@@ -62,9 +75,12 @@ template <typename K, int KIND>
 void launch_onesweep(int cfg, bool descending, bool values, bool wide_status,
                      const onesweep_args& a, hipStream_t s);
 
+//! One tile per workgroup, taken in launch order from an agent-scope
+//! counter, so a tile only waits on tiles that are already running (no
+//! co-residency is assumed).
 template <typename K, typename S, int KIND, bool DESC, bool VALUES, int BLOCK,
-          int ITEMS, bool SPLIT>
-__global__ __launch_bounds__(BLOCK) void onesweep_kernel(
+          int ITEMS, bool SPLIT, int WPE>
+__global__ __launch_bounds__(BLOCK, WPE) void onesweep_kernel(
     K* keys_a, K* keys_b, uint32_t* vals_a, uint32_t* vals_b, uint32_t n,
     uint32_t pass, const uint32_t* __restrict__ trivial,
     const uint32_t* __restrict__ digit_base, S* status, S* status_next,
@@ -123,7 +139,8 @@ template <typename K, typename S, int KIND, bool DESC, bool VALUES, int C>
 void launch(const onesweep_args& a, hipStream_t s)
 {
     constexpr int BLOCK = OS_BLOCK[C], ITEMS = OS_ITEMS[C];
-    auto kernel = onesweep_kernel<K, S, KIND, DESC, VALUES, BLOCK, ITEMS, OS_SPLIT[C]>;
+    auto kernel = onesweep_kernel<K, S, KIND, DESC, VALUES, BLOCK, ITEMS, OS_SPLIT[C],
+                                  OS_WAVES[C]>;
     const std::size_t lds = os_lds(C, sizeof(K), VALUES);
     static bool opted = false; // benign race: the attribute call is idempotent
     if (!opted && lds > 65536)
@@ -178,6 +195,19 @@ void launch_onesweep(int cfg, bool descending, bool values, bool wide_status,
     }
 #undef CLC_OS
 }
+
+// Instantiated once each, in radix_os_<type>.hip; other translation units
+// only call them.
+#define CLC_OS_EXTERN(K, KIND)                                                       \
+    extern template void launch_onesweep<K, KIND>(int, bool, bool, bool,             \
+                                                  const onesweep_args&, hipStream_t)
+CLC_OS_EXTERN(uint32_t, dev::KEY_UNSIGNED);
+CLC_OS_EXTERN(uint32_t, dev::KEY_SIGNED);
+CLC_OS_EXTERN(uint32_t, dev::KEY_FLOAT);
+CLC_OS_EXTERN(uint64_t, dev::KEY_UNSIGNED);
+CLC_OS_EXTERN(uint64_t, dev::KEY_SIGNED);
+CLC_OS_EXTERN(uint64_t, dev::KEY_FLOAT);
+#undef CLC_OS_EXTERN
 
 } // namespace radix
 } // namespace clc

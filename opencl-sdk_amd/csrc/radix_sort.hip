@@ -45,18 +45,25 @@ using namespace radix;
 
 // ---- upfront histogram of every digit place -------------------------------
 
+//! Every bin has C = 32 KiB / (P KiB) copies, interleaved (copy = lane mod
+//! C, adjacent copies in adjacent banks): lanes that hit the same digit,
+//! the common case for skewed keys, serialise at most 64 / C deep instead
+//! of 64.
 template <typename K, int KIND, bool DESC, int BLOCK, bool VEC>
 __global__ __launch_bounds__(BLOCK) void digit_histogram_kernel(
     const K* __restrict__ keys, uint32_t n, uint32_t* __restrict__ hist)
 {
     constexpr int P = sizeof(K);
-    __shared__ uint32_t s_hist[P * RADIX];
-    for (uint32_t i = threadIdx.x; i < P * RADIX; i += BLOCK) s_hist[i] = 0;
+    constexpr int C = 32 / P;
+    __shared__ uint32_t s_hist[P * RADIX * C];
+    for (uint32_t i = threadIdx.x; i < P * RADIX * C; i += BLOCK) s_hist[i] = 0;
     __syncthreads();
+    const uint32_t copy = threadIdx.x % C;
     auto count = [&](K k) {
 #pragma unroll
         for (int p = 0; p < P; ++p)
-            atomicAdd(&s_hist[p * RADIX + digit_of<K, KIND, DESC>(k, 8u * p)], 1u);
+            atomicAdd(&s_hist[(p * RADIX + digit_of<K, KIND, DESC>(k, 8u * p)) * C + copy],
+                      1u);
     };
     const uint32_t stride = gridDim.x * BLOCK;
     uint32_t start = 0;
@@ -78,7 +85,12 @@ __global__ __launch_bounds__(BLOCK) void digit_histogram_kernel(
         count(keys[i]);
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < P * RADIX; i += BLOCK)
-        if (s_hist[i] != 0) atomicAdd(&hist[i], s_hist[i]);
+    {
+        uint32_t c = 0;
+#pragma unroll
+        for (int j = 0; j < C; ++j) c += s_hist[i * C + j];
+        if (c != 0) atomicAdd(&hist[i], c);
+    }
 }
 
 //! base[p][d] = sum of hist[p][d'] for d' < d; one block per digit place.
@@ -147,7 +159,7 @@ int pick_shape(context& ctx, std::size_t kbytes, bool values)
         tuned[detail::sort_tuning_slot(kbytes, values)] = stored;
         return stored[0];
     }
-    return 0;
+    return os_default(kbytes, values);
 }
 
 template <typename K, int KIND>

@@ -53,20 +53,41 @@ template <typename K, int KIND, bool DESC> struct key_digit
 
 // ---- one digit place --------------------------------------------------------
 
-//! One tile of one digit place. FULL tiles skip every bounds check; only
-//! the last tile of the array runs the guarded form. With SPLIT, keys and
-//! values are reordered one after the other through the same LDS buffer
-//! (s_vals unused; each thread keeps its slots and destinations in
+//! Loads one tile's keys into registers, wave-striped: round r of a wave
+//! covers 64 consecutive keys, so ranking round by round ranks in input
+//! order. In the guarded (last) tile, lanes past n get 0.
+template <typename K, int BLOCK, int ITEMS, bool FULL>
+__device__ __forceinline__ void onesweep_load(const K* __restrict__ in, uint32_t n,
+                                              uint32_t tile, K (&x)[ITEMS])
+{
+    constexpr uint32_t TILE = BLOCK * ITEMS;
+    const uint32_t lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+    const uint32_t wave_base = tile * TILE + wave * (WAVE * ITEMS);
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r)
+    {
+        const uint32_t idx = wave_base + r * WAVE + lane;
+        if constexpr (FULL)
+            x[r] = in[idx];
+        else
+            x[r] = idx < n ? in[idx] : K(0);
+    }
+}
+
+//! One tile of one digit place whose keys (and values) are already in
+//! registers (k, v, loaded by onesweep_load). FULL tiles skip every bounds
+//! check; only the last tile of the array runs the guarded form. With SPLIT,
+//! keys and values are reordered one after the other through the same LDS
+//! buffer (s_vals unused; each thread keeps its slots and destinations in
 //! registers), which halves the staging LDS of key-value tiles so larger
 //! tiles (longer contiguous digit runs in HBM) fit.
 template <typename K, typename S, bool VALUES, int BLOCK, int ITEMS, bool FULL,
           bool SPLIT, typename DF>
-__device__ __forceinline__ void onesweep_tile(
-    const K* __restrict__ keys_in, K* __restrict__ keys_out,
-    const uint32_t* __restrict__ vals_in, uint32_t* __restrict__ vals_out,
-    uint32_t n, DF dig, const uint32_t* __restrict__ digit_base,
-    S* status, uint32_t tile, uint32_t* s_hist, K* s_keys, uint32_t* s_vals,
-    uint32_t* s_gbase, uint32_t* s_tmp)
+__device__ __forceinline__ void onesweep_rank_scatter(
+    K (&k)[ITEMS], uint32_t (&v)[VALUES ? ITEMS : 1], K* __restrict__ keys_out,
+    uint32_t* __restrict__ vals_out, uint32_t n, DF dig,
+    const uint32_t* __restrict__ digit_base, S* status, uint32_t tile, uint32_t* s_hist,
+    K* s_keys, uint32_t* s_vals, uint32_t* s_gbase, uint32_t* s_tmp)
 {
     using SB = status_bits<S>;
     constexpr int W = BLOCK / WAVE;
@@ -74,25 +95,6 @@ __device__ __forceinline__ void onesweep_tile(
     const uint32_t tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
     const uint32_t tile_base = tile * TILE;
     const uint32_t wave_base = tile_base + wave * (WAVE * ITEMS);
-
-    // Wave-striped loads: round r of a wave covers 64 consecutive keys.
-    K k[ITEMS];
-    uint32_t v[VALUES ? ITEMS : 1];
-#pragma unroll
-    for (int r = 0; r < ITEMS; ++r)
-    {
-        const uint32_t idx = wave_base + r * WAVE + lane;
-        if constexpr (FULL)
-        {
-            k[r] = keys_in[idx];
-            if constexpr (VALUES) v[r] = vals_in[idx];
-        }
-        else
-        {
-            k[r] = idx < n ? keys_in[idx] : K(0);
-            if constexpr (VALUES) v[r] = idx < n ? vals_in[idx] : 0u;
-        }
-    }
 
     // Rank inside the wave: round by round, each key's rank among equal
     // digits is the per-wave running count plus its lane-order position in
@@ -182,7 +184,41 @@ __device__ __forceinline__ void onesweep_tile(
     // Write the reordered tile: consecutive threads take consecutive slots,
     // which land in contiguous runs of one digit each.
     const uint32_t count = FULL ? TILE : n - tile_base;
-    if constexpr (SPLIT && VALUES)
+    if constexpr (SPLIT && VALUES && sizeof(K) == 8)
+    {
+        // An 8-byte slot holds two words once its key is written out: the
+        // key's destination (low) and, after the value pass, the value that
+        // belongs there (high). No destinations stay in registers.
+        uint32_t* s_w = reinterpret_cast<uint32_t*>(s_keys);
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j)
+        {
+            const uint32_t i = j * BLOCK + tid;
+            if (FULL || i < count)
+            {
+                const K key = s_keys[i];
+                const uint32_t dst = s_gbase[dig(key)] + i;
+                keys_out[dst] = key;
+                s_w[2 * i] = dst;
+            }
+        }
+        __syncthreads(); // every destination is in LDS
+#pragma unroll
+        for (int r = 0; r < ITEMS; ++r)
+            if (FULL || wave_base + r * WAVE + lane < n) s_w[2 * rank[r] + 1] = v[r];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j)
+        {
+            const uint32_t i = j * BLOCK + tid;
+            if (FULL || i < count)
+            {
+                const uint2 dv = reinterpret_cast<const uint2*>(s_keys)[i];
+                vals_out[dv.x] = dv.y;
+            }
+        }
+    }
+    else if constexpr (SPLIT && VALUES)
     {
         uint32_t dst[ITEMS];
 #pragma unroll
@@ -224,6 +260,25 @@ __device__ __forceinline__ void onesweep_tile(
             }
         }
     }
+}
+
+//! One tile of one digit place, loads included (the non-persistent form).
+template <typename K, typename S, bool VALUES, int BLOCK, int ITEMS, bool FULL,
+          bool SPLIT, typename DF>
+__device__ __forceinline__ void onesweep_tile(
+    const K* __restrict__ keys_in, K* __restrict__ keys_out,
+    const uint32_t* __restrict__ vals_in, uint32_t* __restrict__ vals_out,
+    uint32_t n, DF dig, const uint32_t* __restrict__ digit_base,
+    S* status, uint32_t tile, uint32_t* s_hist, K* s_keys, uint32_t* s_vals,
+    uint32_t* s_gbase, uint32_t* s_tmp)
+{
+    K k[ITEMS];
+    uint32_t v[VALUES ? ITEMS : 1];
+    onesweep_load<K, BLOCK, ITEMS, FULL>(keys_in, n, tile, k);
+    if constexpr (VALUES) onesweep_load<uint32_t, BLOCK, ITEMS, FULL>(vals_in, n, tile, v);
+    onesweep_rank_scatter<K, S, VALUES, BLOCK, ITEMS, FULL, SPLIT>(
+        k, v, keys_out, vals_out, n, dig, digit_base, status, tile, s_hist, s_keys, s_vals,
+        s_gbase, s_tmp);
 }
 
 // ---- whole segment in LDS ----------------------------------------------------
@@ -282,10 +337,12 @@ __device__ __forceinline__ void digit_bases(uint32_t* s_hist, uint32_t* s_tmp)
 }
 
 //! Stable sort of m <= BLOCK * ITEMS keys (and values) by a single block,
-//! entirely in LDS: sizeof(K) passes of 8-bit digits, each a wave-ballot
-//! rank plus an LDS reorder, then one contiguous write back to src/dst.
-//! Keys past m take digit 255 in every pass; they sit at the end of the
-//! tile, so stability keeps them behind every real key.
+//! entirely in LDS: one pass per 8-bit digit place that is not the same for
+//! every key of the segment (found by an OR/AND reduction of the keys'
+//! order images), each a wave-ballot rank plus an LDS reorder, then one
+//! contiguous write back (none if no place varies). Keys past m take digit
+//! 255 in every pass; they sit at the end of the tile, so stability keeps
+//! them behind every real key.
 template <typename K, int KIND, bool DESC, bool VALUES, int BLOCK, int ITEMS>
 __device__ __forceinline__ void lds_sort_segment(
     K* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t m,
@@ -297,19 +354,51 @@ __device__ __forceinline__ void lds_sort_segment(
     const uint32_t wave_base = wave * (WAVE * ITEMS);
     uint32_t* my_hist = s_hist + wave * RADIX;
 
+    __shared__ unsigned long long s_or, s_and;
+    if (tid == 0)
+    {
+        s_or = 0;
+        s_and = ~0ull;
+    }
     K k[ITEMS];
     uint32_t v[VALUES ? ITEMS : 1];
+    unsigned long long b_or = 0, b_and = ~0ull;
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r)
     {
         const uint32_t pos = wave_base + r * WAVE + lane;
         k[r] = pos < m ? keys[pos] : K(0);
         if constexpr (VALUES) v[r] = pos < m ? vals[pos] : 0u;
+        if (pos < m)
+        {
+            const unsigned long long o = order_bits<K, KIND, DESC>(k[r]);
+            b_or |= o;
+            b_and &= o;
+        }
     }
+    // Digit places where every key of the segment agrees (OR == AND on the
+    // byte) would leave the order unchanged: they are skipped.
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1)
+    {
+        b_or |= __shfl_xor(b_or, o, WAVE);
+        b_and &= __shfl_xor(b_and, o, WAVE);
+    }
+    __syncthreads(); // s_or / s_and initialised
+    if (lane == 0)
+    {
+        atomicOr(&s_or, b_or);
+        atomicAnd(&s_and, b_and);
+    }
+    __syncthreads();
+    const unsigned long long varying = s_or ^ s_and;
+    bool sorted_any = false;
 #pragma unroll 1
     for (int p = 0; p < P; ++p)
     {
         const uint32_t shift = 8u * p;
+        if (((varying >> shift) & 0xFFu) == 0) continue; // uniform across the block
+        sorted_any = true;
         for (uint32_t i = tid; i < W * RADIX; i += BLOCK) s_hist[i] = 0;
         __syncthreads();
         uint32_t rank[ITEMS];
@@ -336,22 +425,27 @@ __device__ __forceinline__ void lds_sort_segment(
             if constexpr (VALUES) s_vals[slot] = v[r];
         }
         __syncthreads();
-        if (p + 1 < P)
-        {
 #pragma unroll
-            for (int r = 0; r < ITEMS; ++r)
-            {
-                const uint32_t pos = wave_base + r * WAVE + lane;
-                k[r] = s_keys[pos];
-                if constexpr (VALUES) v[r] = s_vals[pos];
-            }
-            __syncthreads();
+        for (int r = 0; r < ITEMS; ++r)
+        {
+            const uint32_t pos = wave_base + r * WAVE + lane;
+            k[r] = s_keys[pos];
+            if constexpr (VALUES) v[r] = s_vals[pos];
         }
+        __syncthreads();
     }
-    for (uint32_t i = tid; i < m; i += BLOCK)
+    if (sorted_any)
     {
-        keys[i] = s_keys[i];
-        if constexpr (VALUES) vals[i] = s_vals[i];
+#pragma unroll
+        for (int r = 0; r < ITEMS; ++r)
+        {
+            const uint32_t pos = wave_base + r * WAVE + lane;
+            if (pos < m)
+            {
+                keys[pos] = k[r];
+                if constexpr (VALUES) vals[pos] = v[r];
+            }
+        }
     }
     __syncthreads();
 }
