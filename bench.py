@@ -86,10 +86,13 @@ def check_sorted_pairs(torch, keys0, keys, vals, wide):
     return ok and bool((vals[1:][ties] > vals[:-1][ties]).all())
 
 
-def plain_sort(torch, clc_hip, ctx, wide, log2n, steps, warmup, traffic_dir=None):
-    """Times clc_sort on uniform keys; returns the measurements."""
+def plain_sort(torch, clc_hip, ctx, wide, log2n, steps, warmup, traffic_dir=None,
+               values=True):
+    """Times clc_sort on uniform keys (with index values unless values is
+    False); returns the measurements."""
     n = 1 << log2n
     kbytes = 8 if wide else 4
+    vbytes = 4 if values else 0
     passes = 8 if wide else 4
     executed = 6 if wide else 4  # u64 keys below 2^48: the top two bytes are constant
     g = torch.Generator(device="cuda")
@@ -99,14 +102,18 @@ def plain_sort(torch, clc_hip, ctx, wide, log2n, steps, warmup, traffic_dir=None
     else:
         keys0 = torch.randint(-(1 << 31), (1 << 31) - 1, (n,), dtype=torch.int32,
                               device="cuda", generator=g)
-    vals0 = torch.arange(n, dtype=torch.int32, device="cuda")
+    vals0 = torch.arange(n, dtype=torch.int32, device="cuda") if values else None
     keys = torch.empty_like(keys0)
-    vals = torch.empty_like(vals0)
+    vals = torch.empty_like(vals0) if values else None
     kt = clc_hip.U64 if wide else clc_hip.U32
 
-    for _ in range(warmup):
+    def restore():
         keys.copy_(keys0)
-        vals.copy_(vals0)
+        if values:
+            vals.copy_(vals0)
+
+    for _ in range(warmup):
+        restore()
         ctx.sort(keys, vals, key_type=kt)
     torch.cuda.synchronize()
 
@@ -116,8 +123,7 @@ def plain_sort(torch, clc_hip, ctx, wide, log2n, steps, warmup, traffic_dir=None
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(steps):
-        keys.copy_(keys0)
-        vals.copy_(vals0)
+        restore()
         starts[i].record()
         ctx.sort(keys, vals, key_type=kt)
         ends[i].record()
@@ -127,19 +133,25 @@ def plain_sort(torch, clc_hip, ctx, wide, log2n, steps, warmup, traffic_dir=None
     step_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
     ms = statistics.fmean(step_ms)
     timings = ctx.kernel_timings()
-    ok = check_sorted_pairs(torch, keys0, keys, vals, wide)
+    if values:
+        ok = check_sorted_pairs(torch, keys0, keys, vals, wide)
+    else:
+        k = keys if wide else keys.to(torch.int64) & 0xFFFFFFFF
+        k0 = keys0 if wide else keys0.to(torch.int64) & 0xFFFFFFFF
+        ok = bool((k[1:] >= k[:-1]).all()) and bool(torch.equal(torch.sort(k0).values, k))
 
-    one = timings.get("radix_onesweep_pairs")
+    kernel = "radix_onesweep_pairs" if values else "radix_onesweep_keys"
+    one = timings.get(kernel)
     roofline = None
     if one and one["launches"]:
         # Skipped (constant-digit) passes launch but return at once: average
         # over the executed launches only.
         ran = one["launches"] * executed // passes
         avg_ms = one["total_ms"] / ran
-        bytes_per_launch = n * 2 * (kbytes + 4)  # read + write of key and value
+        bytes_per_launch = n * 2 * (kbytes + vbytes)  # read + write of key and value
         achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
         traffic = None
-        if traffic_dir is not None:
+        if traffic_dir is not None and values:
             tpath = os.path.join(traffic_dir, f"onesweep_pairs_{'u64' if wide else 'u32'}"
                                               f"_2p{log2n}_traffic.json")
             if os.path.exists(tpath):
@@ -147,11 +159,11 @@ def plain_sort(torch, clc_hip, ctx, wide, log2n, steps, warmup, traffic_dir=None
                     traffic = json.load(f).get("hbm_bytes_per_launch")
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": traffic, "kernel": "radix_onesweep_pairs",
+                    "traffic": traffic, "kernel": kernel,
                     "avg_launch_ms": round(avg_ms, 4),
                     "algorithmic_bytes_per_launch": bytes_per_launch}
-    sort_bytes = n * (kbytes + passes * 2 * (kbytes + 4))  # k + P*2(k+v)
-    executed_bytes = n * (kbytes + executed * 2 * (kbytes + 4))
+    sort_bytes = n * (kbytes + passes * 2 * (kbytes + vbytes))  # k + P*2(k+v)
+    executed_bytes = n * (kbytes + executed * 2 * (kbytes + vbytes))
     del keys0, vals0, keys, vals
     return {
         "n": n, "kbytes": kbytes, "passes": passes, "executed": executed,
@@ -391,6 +403,8 @@ def main() -> int:
     ap.add_argument("--legs", default="u64,learned,segmented",
                     help="extra workloads to report: comma list of u64, learned, "
                          "segmented, or 'none'")
+    ap.add_argument("--keys-only", action="store_true",
+                    help="sort keys without values (main line only)")
     ap.add_argument("--learned-log2n", type=int, default=28)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-log2n", type=int, default=28)
@@ -415,7 +429,7 @@ def main() -> int:
 
     say("main", args.key, f"2^{log2n}")
     m = plain_sort(torch, clc_hip, ctx, wide, log2n, args.steps, args.warmup,
-                   traffic_dir=os.path.join(ROOT, "profiles"))
+                   traffic_dir=os.path.join(ROOT, "profiles"), values=not args.keys_only)
     ok = m["ok"]
     step_ms = m["step_ms"]
     line = {
@@ -431,11 +445,14 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": args.key,
         "data": "synthetic (torch.randint seeded; values 0..n-1)",
-        "config": {"workload": (f"stable key-value radix sort, 2^{log2n} uint64 keys uniform "
-                                "over [0, 2^48) + uint32 index values, ascending") if wide else
-                               (f"stable key-value radix sort, 2^{log2n} uint32 keys "
-                                "uniform over 32 bits + uint32 index values, ascending"),
-                   "n": m["n"], "key_bytes": m["kbytes"], "value_bytes": 4,
+        "config": {"workload": (f"stable {'keys-only' if args.keys_only else 'key-value'} "
+                                f"radix sort, 2^{log2n} "
+                                + ("uint64 keys uniform over [0, 2^48)" if wide else
+                                   "uint32 keys uniform over 32 bits")
+                                + ("" if args.keys_only else " + uint32 index values")
+                                + ", ascending"),
+                   "n": m["n"], "key_bytes": m["kbytes"],
+                   "value_bytes": 0 if args.keys_only else 4,
                    "passes": m["passes"], "executed_passes": m["executed"],
                    "parallelism": "single GPU"},
         "roofline": m["roofline"],

@@ -15,10 +15,10 @@ namespace radix {
 //! while the other waits on its look-back or on HBM). The sort tuner picks
 //! one per key width and values flag; os_default() is used untuned.
 constexpr int NOS = 8;
-constexpr int OS_BLOCK[NOS] = { 1024, 512, 512, 1024, 512, 512, 1024, 512 };
-constexpr int OS_ITEMS[NOS] = { 16, 16, 20, 16, 12, 16, 20, 12 };
-constexpr bool OS_SPLIT[NOS] = { true, false, true, false, false, true, true, true };
-constexpr int OS_WAVES[NOS] = { 4, 4, 4, 4, 4, 4, 4, 6 };
+constexpr int OS_BLOCK[NOS] = { 1024, 1024, 1024, 1024, 512, 512, 1024, 1024 };
+constexpr int OS_ITEMS[NOS] = { 16, 24, 28, 16, 24, 16, 20, 12 };
+constexpr bool OS_SPLIT[NOS] = { true, true, true, false, true, true, true, true };
+constexpr int OS_WAVES[NOS] = { 4, 4, 4, 4, 4, 4, 4, 4 };
 //! LDS a workgroup may use on gfx950.
 constexpr std::size_t OS_LDS_LIMIT = 160 * 1024;
 
@@ -34,10 +34,14 @@ inline std::size_t os_lds(int cfg, std::size_t kbytes, bool values)
     return std::size_t(OS_BLOCK[cfg] / 64) * 1024 + RADIX * 4 + stage + 64;
 }
 
+//! Static LDS of the pass kernel (digit totals, scan scratch), with margin.
+constexpr std::size_t OS_STATIC_LDS = 2304;
+
 //! Whether shape cfg fits in LDS for this key width (shape 0 always does).
 inline bool os_feasible(int cfg, std::size_t kbytes, bool values)
 {
-    return cfg >= 0 && cfg < NOS && os_lds(cfg, kbytes, values) <= OS_LDS_LIMIT;
+    return cfg >= 0 && cfg < NOS
+        && os_lds(cfg, kbytes, values) + OS_STATIC_LDS <= OS_LDS_LIMIT;
 }
 
 //! Untuned shape: the largest tiles that fit one workgroup per CU. Measured
@@ -77,7 +81,10 @@ void launch_onesweep(int cfg, bool descending, bool values, bool wide_status,
 
 //! One tile per workgroup, taken in launch order from an agent-scope
 //! counter, so a tile only waits on tiles that are already running (no
-//! co-residency is assumed).
+//! co-residency is assumed). Not persistent on purpose: on gfx9 a wave's
+//! stores and loads retire in one in-order count, so a workgroup that went
+//! on to a next tile would wait for its previous tile's stores before it
+//! could use the new keys (measured 20% slower than fresh workgroups).
 template <typename K, typename S, int KIND, bool DESC, bool VALUES, int BLOCK,
           int ITEMS, bool SPLIT, int WPE>
 __global__ __launch_bounds__(BLOCK, WPE) void onesweep_kernel(
@@ -98,7 +105,18 @@ __global__ __launch_bounds__(BLOCK, WPE) void onesweep_kernel(
     __shared__ uint32_t s_tile;
 
     const uint32_t tid = threadIdx.x;
-    if (trivial[pass] != 0)
+    // The tile index and every place's flag are requested together (a
+    // skipped pass wastes an index of its own counter, which nothing reads).
+    if (tid == 0) s_tile = fetch_add_agent(tile_counter, 1u);
+    uint32_t skip = 0; // bit p: digit place p is constant
+#pragma unroll
+    for (int q = 0; q < static_cast<int>(sizeof(K)) / 4; ++q)
+    {
+        const uint4 f = reinterpret_cast<const uint4*>(trivial)[q];
+        skip |= ((f.x != 0 ? 1u : 0u) | (f.y != 0 ? 2u : 0u) | (f.z != 0 ? 4u : 0u)
+                 | (f.w != 0 ? 8u : 0u)) << (4 * q);
+    }
+    if ((skip >> pass) & 1u)
     {
         // Every key has the same digit here: the pass would be the identity.
         // Only clear the next pass's status words (one row per workgroup).
@@ -106,14 +124,13 @@ __global__ __launch_bounds__(BLOCK, WPE) void onesweep_kernel(
             status_next[static_cast<size_t>(blockIdx.x) * RADIX + tid] = 0;
         return;
     }
-    uint32_t parity = 0; // executed passes before this one, mod 2
-    for (uint32_t p = 0; p < pass; ++p) parity ^= trivial[p] == 0 ? 1u : 0u;
+    // Executed passes before this one, mod 2, choose the buffers.
+    const uint32_t parity = __popc(~skip & ((1u << pass) - 1u)) & 1u;
     const K* keys_in = parity ? keys_b : keys_a;
     K* keys_out = parity ? keys_a : keys_b;
     const uint32_t* vals_in = parity ? vals_b : vals_a;
     uint32_t* vals_out = parity ? vals_a : vals_b;
 
-    if (tid == 0) s_tile = fetch_add_agent(tile_counter, 1u);
     for (uint32_t i = tid; i < W * RADIX; i += BLOCK) s_hist[i] = 0;
     __syncthreads();
     const uint32_t tile = s_tile;

@@ -32,11 +32,17 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d)
 #pragma unroll
     for (int b = 0; b < 8; ++b)
     {
-        const uint32_t m =
-            static_cast<uint32_t>(static_cast<int32_t>(d << (31 - b)) >> 31);
+        uint32_t m = static_cast<uint32_t>(static_cast<int32_t>(d << (31 - b)) >> 31);
+        // Opaque to the optimiser, so the ballot compares m itself (one
+        // v_bfe_i32 + one v_cmp per bit, instead of re-deriving the bit with
+        // a second shift) and each half folds into one three-input op.
+        asm("" : "+v"(m));
         const uint64_t bal = __ballot(m != 0);
         diff_lo |= static_cast<uint32_t>(bal) ^ m;
         diff_hi |= static_cast<uint32_t>(bal >> 32) ^ m;
+        // Keeps the accumulation a chain of diff | (bal ^ m) steps, each one
+        // v_bitop3, rather than XORs re-associated into OR trees.
+        asm("" : "+v"(diff_lo), "+v"(diff_hi));
     }
     return ~((static_cast<uint64_t>(diff_hi) << 32) | diff_lo);
 }
@@ -52,6 +58,62 @@ template <typename K, int KIND, bool DESC> struct key_digit
 };
 
 // ---- one digit place --------------------------------------------------------
+
+//! Decoupled look-back of one tile over a window of predecessors. Wave w
+//! owns digits [w * DPW, (w + 1) * DPW); lane (dd, j) = (lane / J, lane % J)
+//! reads digit w * DPW + dd of the tile `consumed + j + 1` places back, so
+//! one round of parallel loads covers J = BLOCK / 256 predecessors of every
+//! digit (a single-thread walk would take J round trips). A digit sums the
+//! words before the first unpublished one and stops at its first PREFIX;
+//! it then publishes this tile's inclusive PREFIX (total from s_total) and
+//! adds its exclusive prefix to s_gbase. Every wave of the block calls it.
+template <typename S, int BLOCK>
+__device__ __forceinline__ void lookback_window(S* status, uint32_t tile,
+                                                const uint32_t* s_total, uint32_t* s_gbase)
+{
+    using SB = status_bits<S>;
+    constexpr uint32_t J = BLOCK / RADIX;
+    constexpr uint32_t DPW = WAVE / J;
+    static_assert(J * RADIX == BLOCK && (J == 1 || J == 2 || J == 4), "block of 256/512/1024");
+    const uint32_t lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+    const uint32_t dd = lane / J, j = lane % J;
+    const uint32_t d = wave * DPW + dd;
+    S excl = 0;
+    uint32_t consumed = 0; // predecessors already summed; the same in a digit's J lanes
+    bool done = tile == 0;
+    while (__ballot(!done) != 0)
+    {
+        S s = SB::PREFIX; // before tile 0: an empty prefix
+        const uint32_t back = consumed + j + 1;
+        if (!done && back <= tile)
+            s = load_agent(status + static_cast<size_t>(tile - back) * RADIX + d);
+        const uint64_t pm = __ballot(!done && (s & SB::PREFIX) != 0);
+        const uint64_t zm = __ballot(!done && (s & SB::FLAGS) == 0);
+        const uint32_t gp = static_cast<uint32_t>(pm >> (dd * J)) & ((1u << J) - 1);
+        const uint32_t gz = static_cast<uint32_t>(zm >> (dd * J)) & ((1u << J) - 1);
+        const uint32_t jp = gp ? __builtin_ctz(gp) : J; // first PREFIX in the window
+        const uint32_t jz = gz ? __builtin_ctz(gz) : J; // first unpublished word
+        const uint32_t take = jz < jp ? jz : (jp < J ? jp + 1 : J);
+        S part = j < take ? (s & SB::VALUE) : S(0);
+        if constexpr (J >= 2) part += __shfl_xor(part, 1);
+        if constexpr (J >= 4) part += __shfl_xor(part, 2);
+        if (!done)
+        {
+            excl += part;
+            consumed += take;
+            done = jp < jz;
+        }
+        // A digit stopped at an unpublished word: back off briefly.
+        if (__ballot(!done && jz < J) != 0) __builtin_amdgcn_s_sleep(1);
+    }
+    if (j == 0)
+    {
+        if (tile > 0)
+            store_agent(status + static_cast<size_t>(tile) * RADIX + d,
+                        SB::PREFIX | (excl + S(s_total[d])));
+        s_gbase[d] += static_cast<uint32_t>(excl);
+    }
+}
 
 //! Loads one tile's keys into registers, wave-striped: round r of a wave
 //! covers 64 consecutive keys, so ranking round by round ranks in input
@@ -78,9 +140,10 @@ __device__ __forceinline__ void onesweep_load(const K* __restrict__ in, uint32_t
 //! registers (k, v, loaded by onesweep_load). FULL tiles skip every bounds
 //! check; only the last tile of the array runs the guarded form. With SPLIT,
 //! keys and values are reordered one after the other through the same LDS
-//! buffer (s_vals unused; each thread keeps its slots and destinations in
-//! registers), which halves the staging LDS of key-value tiles so larger
-//! tiles (longer contiguous digit runs in HBM) fit.
+//! buffer (s_vals unused; each thread keeps its slots, and for 32-bit keys
+//! the digits of the slots it writes, in registers), which halves the
+//! staging LDS of key-value tiles so larger tiles (longer contiguous digit
+//! runs in HBM) fit.
 template <typename K, typename S, bool VALUES, int BLOCK, int ITEMS, bool FULL,
           bool SPLIT, typename DF>
 __device__ __forceinline__ void onesweep_rank_scatter(
@@ -120,6 +183,7 @@ __device__ __forceinline__ void onesweep_rank_scatter(
     __syncthreads();
 
     // Per digit (thread d): tile total, wave-exclusive prefixes, publish.
+    __shared__ uint32_t s_total[RADIX];
     uint32_t total = 0;
     if (tid < RADIX)
     {
@@ -138,6 +202,8 @@ __device__ __forceinline__ void onesweep_rank_scatter(
     {
 #pragma unroll
         for (int w = 0; w < W; ++w) s_hist[w * RADIX + tid] += digit_start;
+        s_total[tid] = total;
+        s_gbase[tid] = digit_base[tid] - digit_start; // the look-back adds the prefix
     }
     __syncthreads();
 
@@ -154,31 +220,12 @@ __device__ __forceinline__ void onesweep_rank_scatter(
             if constexpr (SPLIT) rank[r] = slot; // kept for the value pass
         }
     }
+    // Nothing reads this wave's counters any more: clear them for the tile
+    // a persistent caller processes next.
+#pragma unroll
+    for (int q = 0; q < static_cast<int>(RADIX / WAVE); ++q) my_hist[q * WAVE + lane] = 0;
 
-    // Decoupled look-back (thread d walks digit d's chain backwards).
-    if (tid < RADIX)
-    {
-        S excl = 0;
-        if (tile > 0)
-        {
-            const S* p = status + static_cast<size_t>(tile - 1) * RADIX + tid;
-            for (;;)
-            {
-                const S s = load_agent(p);
-                if ((s & SB::FLAGS) == 0) // not published yet: back off briefly
-                {
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                excl += s & SB::VALUE;
-                if (s & SB::PREFIX) break;
-                p -= RADIX;
-            }
-            store_agent(status + static_cast<size_t>(tile) * RADIX + tid,
-                        SB::PREFIX | (excl + S(total)));
-        }
-        s_gbase[tid] = digit_base[tid] + static_cast<uint32_t>(excl) - digit_start;
-    }
+    lookback_window<S, BLOCK>(status, tile, s_total, s_gbase);
     __syncthreads();
 
     // Write the reordered tile: consecutive threads take consecutive slots,
@@ -220,7 +267,11 @@ __device__ __forceinline__ void onesweep_rank_scatter(
     }
     else if constexpr (SPLIT && VALUES)
     {
-        uint32_t dst[ITEMS];
+        // A thread writes the same slots (j * BLOCK + tid) in both loops:
+        // it keeps their digits, four to a register, for the value pass.
+        uint32_t dg[(ITEMS + 3) / 4];
+#pragma unroll
+        for (int q = 0; q < (ITEMS + 3) / 4; ++q) dg[q] = 0;
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j)
         {
@@ -228,8 +279,9 @@ __device__ __forceinline__ void onesweep_rank_scatter(
             if (FULL || i < count)
             {
                 const K key = s_keys[i];
-                dst[j] = s_gbase[dig(key)] + i;
-                keys_out[dst[j]] = key;
+                const uint32_t d = dig(key);
+                dg[j / 4] |= d << (8 * (j % 4));
+                keys_out[s_gbase[d] + i] = key;
             }
         }
         __syncthreads(); // keys out of the buffer: reuse it for the values
@@ -242,7 +294,8 @@ __device__ __forceinline__ void onesweep_rank_scatter(
         for (int j = 0; j < ITEMS; ++j)
         {
             const uint32_t i = j * BLOCK + tid;
-            if (FULL || i < count) vals_out[dst[j]] = s_v[i];
+            if (FULL || i < count)
+                vals_out[s_gbase[(dg[j / 4] >> (8 * (j % 4))) & 0xFFu] + i] = s_v[i];
         }
     }
     else
@@ -275,6 +328,9 @@ __device__ __forceinline__ void onesweep_tile(
     K k[ITEMS];
     uint32_t v[VALUES ? ITEMS : 1];
     onesweep_load<K, BLOCK, ITEMS, FULL>(keys_in, n, tile, k);
+    // Values are loaded with the keys: loaded later, their latency shows
+    // (measured slower), and any load issued before the look-back delays its
+    // polls (a wave waits for its vector memory operations in issue order).
     if constexpr (VALUES) onesweep_load<uint32_t, BLOCK, ITEMS, FULL>(vals_in, n, tile, v);
     onesweep_rank_scatter<K, S, VALUES, BLOCK, ITEMS, FULL, SPLIT>(
         k, v, keys_out, vals_out, n, dig, digit_base, status, tile, s_hist, s_keys, s_vals,
