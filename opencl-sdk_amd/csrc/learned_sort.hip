@@ -47,7 +47,10 @@ using namespace radix;
 
 constexpr uint32_t LEAVES = 256;
 constexpr uint32_t SAMPLE_STRIDE = 100;
-constexpr int LBLOCK = 512, LITEMS = 16; // onesweep tile of the id passes
+// Onesweep tile of the id passes: 1024 threads x 14 keys with split staging
+// and the digits staged as bytes (the largest tile whose LDS fits 64-bit
+// keys with the model).
+constexpr int LBLOCK = 1024, LITEMS = 14;
 constexpr uint32_t LTILE = LBLOCK * LITEMS;
 
 struct rmi_leaf
@@ -82,8 +85,12 @@ __device__ __forceinline__ uint32_t bucket_of(const rmi_model& m, uint64_t bits)
 
 template <typename K, int KIND, bool DESC> struct id_digit
 {
+    //! Inference is costly: one evaluation per key and pass, the digit
+    //! staged in s_dig for the write-out.
+    static constexpr bool STAGED = true;
     const rmi_model* m;
     uint32_t shift;
+    uint8_t* s_dig;
     __device__ __forceinline__ uint32_t operator()(K k) const
     {
         return (bucket_of(*m, static_cast<uint64_t>(order_bits<K, KIND, DESC>(k))) >> shift)
@@ -109,58 +116,101 @@ __global__ __launch_bounds__(256) void sample_kernel(const K* __restrict__ keys,
             order_bits<K, KIND, DESC>(keys[size_t(i) * SAMPLE_STRIDE + SAMPLE_STRIDE / 2]));
 }
 
-//! Sum of v over the block (all threads get it). s has 16 doubles.
-__device__ double block_sum(double v, double* s)
+//! Least-squares fits use the moments of (x, y) = (key - xref, index -
+//! yref) over units of FIT_UNIT consecutive samples of the sorted sample:
+//! one workgroup per unit, then the units of a model combined in a fixed
+//! order (deterministic, and no workgroup walks a whole skewed leaf).
+constexpr uint32_t FIT_UNIT = 4096;
+constexpr int NMOM = 5; // count, sum x, sum y, sum x^2, sum x*y
+
+//! Moments of samples [lo, hi) of the 256-thread block, reduced in a fixed
+//! order into out[] (thread 0).
+__device__ void unit_moments(const uint64_t* __restrict__ s, uint32_t lo, uint32_t hi,
+                             uint64_t xref, uint32_t yref, double* out)
 {
+    __shared__ double red[4][NMOM];
+    double m[NMOM] = { 0, 0, 0, 0, 0 };
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += 256)
+    {
+        const double x = static_cast<double>(s[i] - xref); // s[i] >= xref: sorted
+        const double y = static_cast<double>(i - yref);
+        m[0] += 1.0;
+        m[1] += x;
+        m[2] += y;
+        m[3] += x * x;
+        m[4] += x * y;
+    }
     const uint32_t lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
 #pragma unroll
-    for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_down(v, o, WAVE);
-    if (lane == 0) s[wave] = v;
+    for (int k = 0; k < NMOM; ++k)
+    {
+#pragma unroll
+        for (int o = WAVE / 2; o > 0; o >>= 1) m[k] += __shfl_down(m[k], o, WAVE);
+        if (lane == 0) red[wave][k] = m[k];
+    }
     __syncthreads();
-    double t = 0;
-    for (uint32_t w = 0; w < blockDim.x / WAVE; ++w) t += s[w];
-    __syncthreads();
-    return t;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < NMOM; ++k) out[k] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
 }
 
-//! Least squares of y on x over [lo, hi) of the sorted sample, y = scale * i.
-__device__ void fit_line(const uint64_t* s, uint32_t lo, uint32_t hi, double scale,
-                         double* red, double& a, double& b)
+//! Sums units [u0, u1) of partial moments in a fixed order (thread 0 gets it).
+__device__ void combine_moments(const double* __restrict__ part, uint32_t u0, uint32_t u1,
+                                double* out)
 {
-    const double cnt = double(hi - lo);
-    double sx = 0, sy = 0;
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x)
+    __shared__ double red[256][NMOM];
+    double m[NMOM] = { 0, 0, 0, 0, 0 };
+    for (uint32_t u = u0 + threadIdx.x; u < u1; u += 256)
+        for (int k = 0; k < NMOM; ++k) m[k] += part[size_t(u) * NMOM + k];
+    for (int k = 0; k < NMOM; ++k) red[threadIdx.x][k] = m[k];
+    __syncthreads();
+    for (uint32_t w = 128; w > 0; w >>= 1)
     {
-        sx += double(s[i]);
-        sy += scale * double(i);
+        if (threadIdx.x < w)
+            for (int k = 0; k < NMOM; ++k) red[threadIdx.x][k] += red[threadIdx.x + w][k];
+        __syncthreads();
     }
-    sx = block_sum(sx, red);
-    sy = block_sum(sy, red);
-    const double mx = sx / cnt, my = sy / cnt;
-    double sxx = 0, sxy = 0;
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x)
-    {
-        const double dx = double(s[i]) - mx;
-        sxx += dx * dx;
-        sxy += dx * (scale * double(i) - my);
-    }
-    sxx = block_sum(sxx, red);
-    sxy = block_sum(sxy, red);
-    a = sxx > 0 ? fmax(sxy / sxx, 0.0) : 0.0;
-    b = my - a * mx;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < NMOM; ++k) out[k] = red[0][k];
 }
 
-__global__ __launch_bounds__(1024) void root_fit_kernel(const uint64_t* __restrict__ s,
-                                                        uint32_t m, uint32_t nb,
-                                                        rmi_model* model)
+//! Slope (>= 0) and intercept of y on x from moments (y = a x + c).
+__device__ void line_from_moments(const double* mo, double& a, double& c)
 {
-    __shared__ double red[16];
-    double a, b;
-    fit_line(s, 0, m, double(LEAVES) / double(m), red, a, b);
+    const double n = mo[0];
+    const double mx = mo[1] / n, my = mo[2] / n;
+    const double var = mo[3] / n - mx * mx;
+    const double cov = mo[4] / n - mx * my;
+    a = var > 0 ? fmax(cov / var, 0.0) : 0.0;
+    c = my - a * mx;
+}
+
+//! Root moments: unit u of the whole sample, relative to its first key.
+__global__ __launch_bounds__(256) void root_partial_kernel(const uint64_t* __restrict__ s,
+                                                           uint32_t m, double* __restrict__ part)
+{
+    const uint32_t lo = blockIdx.x * FIT_UNIT;
+    double mo[NMOM];
+    unit_moments(s, lo, min(m, lo + FIT_UNIT), s[0], 0, mo);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < NMOM; ++k) part[size_t(blockIdx.x) * NMOM + k] = mo[k];
+}
+
+//! Root model: leaf = a0 * x + b0 maps the key to 256 leaves by its
+//! least-squares sample position.
+__global__ __launch_bounds__(256) void root_fit_kernel(const uint64_t* __restrict__ s,
+                                                       const double* __restrict__ part,
+                                                       uint32_t units, uint32_t m, uint32_t nb,
+                                                       rmi_model* model)
+{
+    double mo[NMOM];
+    combine_moments(part, 0, units, mo);
     if (threadIdx.x == 0)
     {
-        model->a0 = a;
-        model->b0 = b;
+        double a, c;
+        line_from_moments(mo, a, c); // index ~ a * (x - s[0]) + c
+        const double scale = double(LEAVES) / double(m);
+        model->a0 = a * scale;
+        model->b0 = (c - a * static_cast<double>(s[0])) * scale;
         model->nb = double(nb);
     }
 }
@@ -181,18 +231,69 @@ __global__ __launch_bounds__(256) void leaf_bounds_kernel(const uint64_t* __rest
     }
 }
 
+//! Units of each leaf: ustart[l] = first unit of leaf l, ustart[LEAVES] =
+//! all units (a leaf of c samples has ceil(c / FIT_UNIT) units).
+__global__ __launch_bounds__(256) void leaf_units_kernel(const uint32_t* __restrict__ lstart,
+                                                         uint32_t* __restrict__ ustart)
+{
+    __shared__ uint32_t tmp[4];
+    const uint32_t l = threadIdx.x;
+    const uint32_t cnt = lstart[l + 1] - lstart[l];
+    const uint32_t u = (cnt + FIT_UNIT - 1) / FIT_UNIT;
+    const uint32_t before = block256_exclusive_add<256>(u, tmp);
+    ustart[l] = before;
+    if (l == LEAVES - 1) ustart[LEAVES] = before + u;
+}
+
+//! Leaf moments: one unit of one leaf, relative to the leaf's first key and
+//! index. Launched with an upper bound of units; extra blocks return.
+__global__ __launch_bounds__(256) void leaf_partial_kernel(const uint64_t* __restrict__ s,
+                                                           const uint32_t* __restrict__ lstart,
+                                                           const uint32_t* __restrict__ ustart,
+                                                           double* __restrict__ part)
+{
+    const uint32_t b = blockIdx.x;
+    if (b >= ustart[LEAVES]) return;
+    uint32_t lo_l = 0, hi_l = LEAVES; // leaf: last l with ustart[l] <= b
+    while (hi_l - lo_l > 1)
+    {
+        const uint32_t mid = (lo_l + hi_l) / 2;
+        if (ustart[mid] <= b) lo_l = mid;
+        else hi_l = mid;
+    }
+    const uint32_t l = lo_l;
+    const uint32_t first = lstart[l];
+    const uint32_t lo = first + (b - ustart[l]) * FIT_UNIT;
+    double mo[NMOM];
+    unit_moments(s, lo, min(lstart[l + 1], lo + FIT_UNIT), s[first], first, mo);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < NMOM; ++k) part[size_t(b) * NMOM + k] = mo[k];
+}
+
+//! Leaf l: linear model of the empirical CDF (slope clamped >= 0) and its
+//! bucket range [lo, hi] (proportional to its share of the sample; each
+//! range ends where the next begins).
 __global__ __launch_bounds__(256) void leaf_fit_kernel(const uint64_t* __restrict__ s,
                                                        uint32_t m, uint32_t nb,
                                                        const uint32_t* __restrict__ lstart,
+                                                       const uint32_t* __restrict__ ustart,
+                                                       const double* __restrict__ part,
                                                        rmi_model* model)
 {
-    __shared__ double red[16];
     const uint32_t l = blockIdx.x;
     const uint32_t lo = lstart[l], hi = lstart[l + 1];
-    double a = 0, b = (double(lo) + double(hi)) / (2.0 * m);
-    if (hi - lo >= 2) fit_line(s, lo, hi, 1.0 / double(m), red, a, b);
+    double mo[NMOM];
+    combine_moments(part, ustart[l], ustart[l + 1], mo);
     if (threadIdx.x == 0)
     {
+        double a = 0, b = (double(lo) + double(hi)) / (2.0 * m);
+        if (hi - lo >= 2)
+        {
+            double ai, ci;
+            line_from_moments(mo, ai, ci); // index - lo ~ ai * (x - s[lo]) + ci
+            a = ai / double(m);
+            b = (double(lo) + ci - ai * static_cast<double>(s[lo])) / double(m);
+        }
         const uint32_t blo = static_cast<uint32_t>(uint64_t(nb) * lo / m);
         const uint32_t next =
             l + 1 < LEAVES ? static_cast<uint32_t>(uint64_t(nb) * hi / m) : nb - 1;
@@ -297,7 +398,7 @@ __global__ __launch_bounds__(LBLOCK) void id_onesweep_kernel(
     rmi_model* s_model = reinterpret_cast<rmi_model*>(s_gbase + RADIX);
     K* s_keys = reinterpret_cast<K*>(reinterpret_cast<unsigned char*>(s_model)
                                      + ((sizeof(rmi_model) + 15) & ~size_t(15)));
-    uint32_t* s_vals = reinterpret_cast<uint32_t*>(s_keys + LTILE);
+    uint8_t* s_dig = reinterpret_cast<uint8_t*>(s_keys + LTILE);
     __shared__ uint32_t s_tmp[4];
     __shared__ uint32_t s_tile;
     const uint32_t tid = threadIdx.x;
@@ -308,13 +409,15 @@ __global__ __launch_bounds__(LBLOCK) void id_onesweep_kernel(
     const uint32_t tile = s_tile;
     if (status_next != nullptr && tid < RADIX)
         status_next[static_cast<size_t>(tile) * RADIX + tid] = 0;
-    const id_digit<K, KIND, DESC> dig{ s_model, shift };
+    const id_digit<K, KIND, DESC> dig{ s_model, shift, s_dig };
+    // Split staging: values pass through the key buffer after the keys.
+    uint32_t* s_vals = reinterpret_cast<uint32_t*>(s_keys);
     if (tile * LTILE + LTILE <= n)
-        onesweep_tile<K, uint32_t, VALUES, LBLOCK, LITEMS, true, false>(
+        onesweep_tile<K, uint32_t, VALUES, LBLOCK, LITEMS, true, true>(
             keys_in, keys_out, vals_in, vals_out, n, dig, digit_base, status, tile, s_hist,
             s_keys, s_vals, s_gbase, s_tmp);
     else
-        onesweep_tile<K, uint32_t, VALUES, LBLOCK, LITEMS, false, false>(
+        onesweep_tile<K, uint32_t, VALUES, LBLOCK, LITEMS, false, true>(
             keys_in, keys_out, vals_in, vals_out, n, dig, digit_base, status, tile, s_hist,
             s_keys, s_vals, s_gbase, s_tmp);
 }
@@ -398,6 +501,11 @@ void run(context& ctx, K* keys, uint32_t* vals, uint32_t n, int key_type, stats*
     const std::size_t o_st1 = L.take(std::size_t(tiles) * RADIX * 4);
     const std::size_t o_model = L.take(sizeof(rmi_model));
     const std::size_t o_lstart = L.take((LEAVES + 1) * 4);
+    const std::size_t o_ustart = L.take((LEAVES + 1) * 4);
+    const uint32_t root_units = (m + FIT_UNIT - 1) / FIT_UNIT;
+    const uint32_t leaf_units_max = LEAVES + root_units; // a leaf adds at most one partial unit
+    const std::size_t o_rpart = L.take(std::size_t(root_units) * NMOM * 8);
+    const std::size_t o_lpart = L.take(std::size_t(leaf_units_max) * NMOM * 8);
     const std::size_t o_base = L.take(2 * RADIX * 4);
     const std::size_t o_sample = L.take(std::size_t(m) * 8);
     const std::size_t o_off = L.take((std::size_t(nb) + 1) * 8);
@@ -414,6 +522,9 @@ void run(context& ctx, K* keys, uint32_t* vals, uint32_t n, int key_type, stats*
                         reinterpret_cast<uint32_t*>(t + o_st1) };
     rmi_model* model = reinterpret_cast<rmi_model*>(t + o_model);
     uint32_t* lstart = reinterpret_cast<uint32_t*>(t + o_lstart);
+    uint32_t* ustart = reinterpret_cast<uint32_t*>(t + o_ustart);
+    double* rpart = reinterpret_cast<double*>(t + o_rpart);
+    double* lpart = reinterpret_cast<double*>(t + o_lpart);
     uint32_t* base = reinterpret_cast<uint32_t*>(t + o_base);
     uint64_t* sample = reinterpret_cast<uint64_t*>(t + o_sample);
     uint64_t* off = reinterpret_cast<uint64_t*>(t + o_off);
@@ -428,11 +539,17 @@ void run(context& ctx, K* keys, uint32_t* vals, uint32_t n, int key_type, stats*
         hipLaunchKernelGGL((sample_kernel<K, KIND, DESC>), dim3(grid), dim3(256), 0, s, keys,
                            m, sample);
         detail::radix_sort(ctx, sample, TY_U64, nullptr, m, false);
-        hipLaunchKernelGGL(root_fit_kernel, dim3(1), dim3(1024), 0, s, sample, m, nb, model);
+        hipLaunchKernelGGL(root_partial_kernel, dim3(root_units), dim3(256), 0, s, sample, m,
+                           rpart);
+        hipLaunchKernelGGL(root_fit_kernel, dim3(1), dim3(256), 0, s, sample, rpart, root_units,
+                           m, nb, model);
         hipLaunchKernelGGL(leaf_bounds_kernel, dim3(grid), dim3(256), 0, s, sample, m, model,
                            lstart);
+        hipLaunchKernelGGL(leaf_units_kernel, dim3(1), dim3(256), 0, s, lstart, ustart);
+        hipLaunchKernelGGL(leaf_partial_kernel, dim3(leaf_units_max), dim3(256), 0, s, sample,
+                           lstart, ustart, lpart);
         hipLaunchKernelGGL(leaf_fit_kernel, dim3(LEAVES), dim3(256), 0, s, sample, m, nb,
-                           lstart, model);
+                           lstart, ustart, lpart, model);
         hipLaunchKernelGGL(sample_runs_kernel, dim3(grid), dim3(256), 0, s, sample, m, model,
                            runs);
         hipLaunchKernelGGL(sample_quality_kernel, dim3(grid), dim3(256), 0, s, sample, m,
@@ -470,7 +587,7 @@ void run(context& ctx, K* keys, uint32_t* vals, uint32_t n, int key_type, stats*
     hipLaunchKernelGGL(id_scan_kernel, dim3(2), dim3(256), 0, s, hist, base);
     const std::size_t lds = std::size_t(LBLOCK / WAVE) * 1024 + RADIX * 4
         + ((sizeof(rmi_model) + 15) & ~std::size_t(15)) + std::size_t(LTILE) * sizeof(K)
-        + (VALUES ? std::size_t(LTILE) * 4 : 0) + 64;
+        + std::size_t(LTILE) + 64;
     auto kernel = id_onesweep_kernel<K, KIND, DESC, VALUES>;
     static bool opted = false;
     if (!opted && lds > 65536)

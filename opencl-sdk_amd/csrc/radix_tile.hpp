@@ -50,6 +50,9 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d)
 //! Digit of a key for one pass: byte shift/8 of the key's order image.
 template <typename K, int KIND, bool DESC> struct key_digit
 {
+    //! Cheap to recompute: no digits are staged in LDS (see
+    //! onesweep_rank_scatter for functors that stage them).
+    static constexpr bool STAGED = false;
     uint32_t shift;
     __device__ __forceinline__ uint32_t operator()(K k) const
     {
@@ -144,6 +147,11 @@ __device__ __forceinline__ void onesweep_load(const K* __restrict__ in, uint32_t
 //! the digits of the slots it writes, in registers), which halves the
 //! staging LDS of key-value tiles so larger tiles (longer contiguous digit
 //! runs in HBM) fit.
+//!
+//! A digit functor with STAGED = true is costly to evaluate: each key's
+//! digit is then computed once, kept (four to a register) for the reorder,
+//! and stored as a byte at its slot in dig.s_dig (TILE bytes of LDS) for
+//! the write-out, instead of being recomputed from the key three times.
 template <typename K, typename S, bool VALUES, int BLOCK, int ITEMS, bool FULL,
           bool SPLIT, typename DF>
 __device__ __forceinline__ void onesweep_rank_scatter(
@@ -162,12 +170,16 @@ __device__ __forceinline__ void onesweep_rank_scatter(
     // Rank inside the wave: round by round, each key's rank among equal
     // digits is the per-wave running count plus its lane-order position in
     // the round. The lowest lane of each digit group advances the counter.
+    constexpr bool STAGED = DF::STAGED;
     uint32_t rank[ITEMS];
+    uint32_t dk[STAGED ? (ITEMS + 3) / 4 : 1]; // digits of k[], four per word
     uint32_t* my_hist = s_hist + wave * RADIX;
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r)
     {
         const uint32_t d = dig(k[r]);
+        if constexpr (STAGED)
+            dk[r / 4] = r % 4 == 0 ? d : dk[r / 4] | (d << (8 * (r % 4)));
         uint64_t peers = match_digit(d);
         bool valid = true;
         if constexpr (!FULL)
@@ -213,8 +225,11 @@ __device__ __forceinline__ void onesweep_rank_scatter(
     {
         if (FULL || wave_base + r * WAVE + lane < n)
         {
-            const uint32_t d = dig(k[r]);
+            uint32_t d;
+            if constexpr (STAGED) d = (dk[r / 4] >> (8 * (r % 4))) & 0xFFu;
+            else d = dig(k[r]);
             const uint32_t slot = my_hist[d] + rank[r];
+            if constexpr (STAGED) dig.s_dig[slot] = static_cast<uint8_t>(d);
             s_keys[slot] = k[r];
             if constexpr (VALUES && !SPLIT) s_vals[slot] = v[r];
             if constexpr (SPLIT) rank[r] = slot; // kept for the value pass
@@ -231,6 +246,10 @@ __device__ __forceinline__ void onesweep_rank_scatter(
     // Write the reordered tile: consecutive threads take consecutive slots,
     // which land in contiguous runs of one digit each.
     const uint32_t count = FULL ? TILE : n - tile_base;
+    auto digit_at = [&](uint32_t i, K key) -> uint32_t {
+        if constexpr (STAGED) return dig.s_dig[i];
+        else return dig(key);
+    };
     if constexpr (SPLIT && VALUES && sizeof(K) == 8)
     {
         // An 8-byte slot holds two words once its key is written out: the
@@ -244,7 +263,7 @@ __device__ __forceinline__ void onesweep_rank_scatter(
             if (FULL || i < count)
             {
                 const K key = s_keys[i];
-                const uint32_t dst = s_gbase[dig(key)] + i;
+                const uint32_t dst = s_gbase[digit_at(i, key)] + i;
                 keys_out[dst] = key;
                 s_w[2 * i] = dst;
             }
@@ -269,9 +288,9 @@ __device__ __forceinline__ void onesweep_rank_scatter(
     {
         // A thread writes the same slots (j * BLOCK + tid) in both loops:
         // it keeps their digits, four to a register, for the value pass.
-        uint32_t dg[(ITEMS + 3) / 4];
+        uint32_t dg[STAGED ? 1 : (ITEMS + 3) / 4];
 #pragma unroll
-        for (int q = 0; q < (ITEMS + 3) / 4; ++q) dg[q] = 0;
+        for (int q = 0; q < (STAGED ? 1 : (ITEMS + 3) / 4); ++q) dg[q] = 0;
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j)
         {
@@ -279,8 +298,8 @@ __device__ __forceinline__ void onesweep_rank_scatter(
             if (FULL || i < count)
             {
                 const K key = s_keys[i];
-                const uint32_t d = dig(key);
-                dg[j / 4] |= d << (8 * (j % 4));
+                const uint32_t d = digit_at(i, key);
+                if constexpr (!STAGED) dg[j / 4] |= d << (8 * (j % 4));
                 keys_out[s_gbase[d] + i] = key;
             }
         }
@@ -295,7 +314,12 @@ __device__ __forceinline__ void onesweep_rank_scatter(
         {
             const uint32_t i = j * BLOCK + tid;
             if (FULL || i < count)
-                vals_out[s_gbase[(dg[j / 4] >> (8 * (j % 4))) & 0xFFu] + i] = s_v[i];
+            {
+                uint32_t d;
+                if constexpr (STAGED) d = dig.s_dig[i];
+                else d = (dg[j / 4] >> (8 * (j % 4))) & 0xFFu;
+                vals_out[s_gbase[d] + i] = s_v[i];
+            }
         }
     }
     else
@@ -307,7 +331,7 @@ __device__ __forceinline__ void onesweep_rank_scatter(
             if (FULL || i < count)
             {
                 const K key = s_keys[i];
-                const uint32_t dst = s_gbase[dig(key)] + i;
+                const uint32_t dst = s_gbase[digit_at(i, key)] + i;
                 keys_out[dst] = key;
                 if constexpr (VALUES) vals_out[dst] = s_vals[i];
             }
