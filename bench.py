@@ -295,6 +295,7 @@ def learned_leg(torch, clc_hip, ctx, log2n, steps, warmup):
             "sample_fit_error": round(st["sample_fit_error"], 5),
             "max_bucket_overfill": round(st["max_bucket_overfill"], 3),
             "fallback_fraction": round(st["fallback_fraction"], 5),
+            "root_log": st["root_log"],
             "distinct_keys_in_first_2^20": distinct_top,
         }
         say(name, out[name])

@@ -264,6 +264,11 @@ typedef struct clc_learned_stats
     /* fraction of keys in buckets too large for LDS, finished by the
      * onesweep fallback instead */
     double fallback_fraction;
+    /* 1 when the model's root is linear in the log image of the key (the
+     * bits of the key converted to double, read as an integer) rather than
+     * in the key: of the two candidate roots, the one whose sample predicts
+     * fewer keys in overfull buckets is used */
+    int root_log;
 } clc_learned_stats;
 
 /* Same result as clc_sort (stable, same key order, values permuted

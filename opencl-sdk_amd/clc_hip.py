@@ -60,7 +60,7 @@ class LearnedStats(ctypes.Structure):
     _fields_ = [("used_model", ctypes.c_int), ("num_buckets", ctypes.c_size_t),
                 ("sample_size", ctypes.c_size_t), ("sample_fit_error", ctypes.c_double),
                 ("max_bucket_overfill", ctypes.c_double),
-                ("fallback_fraction", ctypes.c_double)]
+                ("fallback_fraction", ctypes.c_double), ("root_log", ctypes.c_int)]
 
 
 # Every symbol include/compute_c.h declares, with (restype, argtypes).

@@ -443,6 +443,7 @@ clc_status clc_learned_sort(clc_context ctx, void* keys, clc_type key_type,
             stats->sample_fit_error = st.sample_fit_error;
             stats->max_bucket_overfill = st.max_bucket_overfill;
             stats->fallback_fraction = st.fallback_fraction;
+            stats->root_log = st.root_log;
         }
     });
 }

@@ -48,6 +48,14 @@ __device__ __forceinline__ T fetch_add_agent(T* p, V v)
                                   __HIP_MEMORY_SCOPE_AGENT);
 }
 
+//! Sum over the wave (every lane gets it).
+__device__ __forceinline__ uint32_t wave_sum(uint32_t x)
+{
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, WAVE);
+    return x;
+}
+
 //! Inclusive wave64 prefix sum.
 __device__ __forceinline__ uint32_t wave_inclusive_add(uint32_t x)
 {
@@ -111,6 +119,18 @@ __device__ __forceinline__ K order_bits(K k)
         t = k ^ m;
     }
     if constexpr (DESC) t = ~t;
+    return t;
+}
+
+//! Inverse of order_bits: the key whose image is t.
+template <typename K, int KIND, bool DESC>
+__device__ __forceinline__ K from_order_bits(K t)
+{
+    constexpr int BITS = sizeof(K) * 8;
+    constexpr K TOP = K(1) << (BITS - 1);
+    if constexpr (DESC) t = ~t;
+    if constexpr (KIND == KEY_SIGNED) t ^= TOP;
+    if constexpr (KIND == KEY_FLOAT) t = (t & TOP) ? (t ^ TOP) : ~t;
     return t;
 }
 

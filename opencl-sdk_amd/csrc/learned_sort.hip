@@ -6,7 +6,10 @@
 //                sample, sorted by the plain radix sort
 //   fit          a two-level RMI by reduction kernels: one root linear model
 //                mapping the key to one of 256 leaves, and per leaf a linear
-//                model of the empirical CDF. Slopes are clamped >= 0 and each
+//                model of the empirical CDF. Two candidates are fitted at
+//                once, the root linear in the key or in its log image (keys
+//                spread over many octaves), and the one whose sample
+//                predicts less overfill is kept. Slopes are clamped >= 0 and each
 //                leaf's bucket range ends where the next leaf's begins, so
 //                key -> bucket is a monotone non-decreasing pure function
 //                (conversion to double and IEEE rounding are monotone too):
@@ -14,7 +17,7 @@
 //   check        the sample's predicted buckets estimate how many keys land
 //                in buckets too large for LDS that hold more than one
 //                distinct key; above a threshold the whole input goes to
-//                the plain radix sort (one host read of two numbers)
+//                the plain radix sort (one host read of both candidates' figures)
 //   histogram    one read of the keys: counts of the low and high bytes of
 //                the 16-bit bucket id (inference fused in)
 //   2 passes     stable onesweep passes keyed by the id's low byte, then its
@@ -22,7 +25,7 @@
 //                bucket, in input order inside a bucket. A direct scatter
 //                into 65536 buckets would write one or two keys per bucket
 //                per tile; two 256-way passes write long contiguous runs.
-//   boundaries   one read: exact bucket offsets
+//   boundaries   exact bucket offsets by binary search, one thread per bucket
 //   finish       the segmented sort over the buckets: buckets that fit LDS
 //                are sorted wholly in LDS; larger ones take its onesweep
 //                path (constant digit places skipped per bucket, so buckets
@@ -61,23 +64,40 @@ struct rmi_leaf
 
 struct rmi_model
 {
-    double a0, b0; // leaf ~ a0 * x + b0
+    double a0, b0; // leaf ~ a0 * f + b0, f = root_feature(x)
     double nb; // number of buckets
-    uint32_t pad[2];
+    uint32_t root_log; // root feature: 0 the key, 1 its log image
+    uint32_t pad;
     rmi_leaf leaf[LEAVES];
 };
 
-__device__ __forceinline__ uint32_t leaf_of(const rmi_model& m, double x)
+//! Log image of an order image: the bits of double(x) read as an integer,
+//! a monotone non-decreasing, piecewise linear approximation of
+//! 2^52 * (log2(x) + 1023). A root linear in it spreads keys that span
+//! many octaves (lognormal, Zipf) over the leaves by magnitude.
+__device__ __forceinline__ uint64_t log_image(uint64_t x)
 {
-    const double r = fma(m.a0, x, m.b0);
+    return static_cast<uint64_t>(__double_as_longlong(static_cast<double>(x)));
+}
+
+__device__ __forceinline__ double root_feature(const rmi_model& m, uint64_t bits)
+{
+    return static_cast<double>(m.root_log ? log_image(bits) : bits);
+}
+
+__device__ __forceinline__ uint32_t leaf_of(const rmi_model& m, double f)
+{
+    const double r = fma(m.a0, f, m.b0);
     return r <= 0.0 ? 0u : r >= double(LEAVES - 1) ? LEAVES - 1 : static_cast<uint32_t>(r);
 }
 
-//! Monotone non-decreasing bucket of an order image x.
+//! Monotone non-decreasing bucket of an order image x: the root feature,
+//! its conversion to double and every model step are monotone, leaf slopes
+//! are >= 0 and leaf bucket ranges are ordered.
 __device__ __forceinline__ uint32_t bucket_of(const rmi_model& m, uint64_t bits)
 {
     const double x = static_cast<double>(bits);
-    const rmi_leaf& L = m.leaf[leaf_of(m, x)];
+    const rmi_leaf& L = m.leaf[leaf_of(m, root_feature(m, bits))];
     const double p = fma(L.a, x, L.b) * m.nb;
     const double lo = L.lo, hi = L.hi;
     return p <= lo ? L.lo : p >= hi ? L.hi : static_cast<uint32_t>(p);
@@ -120,19 +140,28 @@ __global__ __launch_bounds__(256) void sample_kernel(const K* __restrict__ keys,
 //! yref) over units of FIT_UNIT consecutive samples of the sorted sample:
 //! one workgroup per unit, then the units of a model combined in a fixed
 //! order (deterministic, and no workgroup walks a whole skewed leaf).
+//!
+//! Two candidate models are fitted side by side (blockIdx.y = c): c = 0 with
+//! the root linear in the key, c = 1 with the root linear in its log image.
+//! Their leaves are both linear in the key. The host keeps the one whose
+//! sample predicts fewer keys in overfull buckets. Per-candidate arrays sit
+//! at a fixed stride from each other.
 constexpr uint32_t FIT_UNIT = 4096;
 constexpr int NMOM = 5; // count, sum x, sum y, sum x^2, sum x*y
+constexpr uint32_t CANDIDATES = 2;
 
 //! Moments of samples [lo, hi) of the 256-thread block, reduced in a fixed
-//! order into out[] (thread 0).
+//! order into out[] (thread 0). With log set, x is the log image of the key
+//! (xref is then a log image too).
 __device__ void unit_moments(const uint64_t* __restrict__ s, uint32_t lo, uint32_t hi,
-                             uint64_t xref, uint32_t yref, double* out)
+                             uint64_t xref, uint32_t yref, bool log, double* out)
 {
     __shared__ double red[4][NMOM];
     double m[NMOM] = { 0, 0, 0, 0, 0 };
     for (uint32_t i = lo + threadIdx.x; i < hi; i += 256)
     {
-        const double x = static_cast<double>(s[i] - xref); // s[i] >= xref: sorted
+        const uint64_t k = log ? log_image(s[i]) : s[i];
+        const double x = static_cast<double>(k - xref); // k >= xref: sorted, monotone image
         const double y = static_cast<double>(i - yref);
         m[0] += 1.0;
         m[1] += x;
@@ -184,47 +213,57 @@ __device__ void line_from_moments(const double* mo, double& a, double& c)
     c = my - a * mx;
 }
 
-//! Root moments: unit u of the whole sample, relative to its first key.
+//! Root moments: unit blockIdx.x of the whole sample, relative to its first
+//! key (or its log image); part has root_units units per candidate.
 __global__ __launch_bounds__(256) void root_partial_kernel(const uint64_t* __restrict__ s,
-                                                           uint32_t m, double* __restrict__ part)
+                                                           uint32_t m, uint32_t root_units,
+                                                           double* __restrict__ part)
 {
+    const bool log = blockIdx.y == 1;
     const uint32_t lo = blockIdx.x * FIT_UNIT;
     double mo[NMOM];
-    unit_moments(s, lo, min(m, lo + FIT_UNIT), s[0], 0, mo);
+    unit_moments(s, lo, min(m, lo + FIT_UNIT), log ? log_image(s[0]) : s[0], 0, log, mo);
     if (threadIdx.x == 0)
-        for (int k = 0; k < NMOM; ++k) part[size_t(blockIdx.x) * NMOM + k] = mo[k];
+        for (int k = 0; k < NMOM; ++k)
+            part[(size_t(blockIdx.y) * root_units + blockIdx.x) * NMOM + k] = mo[k];
 }
 
-//! Root model: leaf = a0 * x + b0 maps the key to 256 leaves by its
-//! least-squares sample position.
+//! Root model: leaf = a0 * f + b0 maps the key's feature f to 256 leaves by
+//! its least-squares sample position.
 __global__ __launch_bounds__(256) void root_fit_kernel(const uint64_t* __restrict__ s,
                                                        const double* __restrict__ part,
                                                        uint32_t units, uint32_t m, uint32_t nb,
-                                                       rmi_model* model)
+                                                       rmi_model* models)
 {
+    const uint32_t c = blockIdx.y;
     double mo[NMOM];
-    combine_moments(part, 0, units, mo);
+    combine_moments(part + size_t(c) * units * NMOM, 0, units, mo);
     if (threadIdx.x == 0)
     {
-        double a, c;
-        line_from_moments(mo, a, c); // index ~ a * (x - s[0]) + c
+        double a, ci;
+        line_from_moments(mo, a, ci); // index ~ a * (f - f(s[0])) + ci
         const double scale = double(LEAVES) / double(m);
-        model->a0 = a * scale;
-        model->b0 = (c - a * static_cast<double>(s[0])) * scale;
-        model->nb = double(nb);
+        const uint64_t fref = c == 1 ? log_image(s[0]) : s[0];
+        rmi_model& M = models[c];
+        M.a0 = a * scale;
+        M.b0 = (ci - a * static_cast<double>(fref)) * scale;
+        M.nb = double(nb);
+        M.root_log = c;
     }
 }
 
 //! lstart[l] = first sample index whose leaf is >= l (leaves are
 //! non-decreasing along the sorted sample); lstart[LEAVES] = m.
 __global__ __launch_bounds__(256) void leaf_bounds_kernel(const uint64_t* __restrict__ s,
-                                                          uint32_t m, const rmi_model* model,
-                                                          uint32_t* __restrict__ lstart)
+                                                          uint32_t m, const rmi_model* models,
+                                                          uint32_t* __restrict__ lstart_all)
 {
+    const rmi_model& M = models[blockIdx.y];
+    uint32_t* lstart = lstart_all + blockIdx.y * (LEAVES + 1);
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < m; i += gridDim.x * 256)
     {
-        const uint32_t li = leaf_of(*model, double(s[i]));
-        const int lp = i == 0 ? -1 : int(leaf_of(*model, double(s[i - 1])));
+        const uint32_t li = leaf_of(M, root_feature(M, s[i]));
+        const int lp = i == 0 ? -1 : int(leaf_of(M, root_feature(M, s[i - 1])));
         for (int l = lp + 1; l <= int(li); ++l) lstart[l] = i;
         if (i == m - 1)
             for (uint32_t l = li + 1; l <= LEAVES; ++l) lstart[l] = m;
@@ -233,10 +272,12 @@ __global__ __launch_bounds__(256) void leaf_bounds_kernel(const uint64_t* __rest
 
 //! Units of each leaf: ustart[l] = first unit of leaf l, ustart[LEAVES] =
 //! all units (a leaf of c samples has ceil(c / FIT_UNIT) units).
-__global__ __launch_bounds__(256) void leaf_units_kernel(const uint32_t* __restrict__ lstart,
-                                                         uint32_t* __restrict__ ustart)
+__global__ __launch_bounds__(256) void leaf_units_kernel(const uint32_t* __restrict__ lstart_all,
+                                                         uint32_t* __restrict__ ustart_all)
 {
     __shared__ uint32_t tmp[4];
+    const uint32_t* lstart = lstart_all + blockIdx.y * (LEAVES + 1);
+    uint32_t* ustart = ustart_all + blockIdx.y * (LEAVES + 1);
     const uint32_t l = threadIdx.x;
     const uint32_t cnt = lstart[l + 1] - lstart[l];
     const uint32_t u = (cnt + FIT_UNIT - 1) / FIT_UNIT;
@@ -246,12 +287,16 @@ __global__ __launch_bounds__(256) void leaf_units_kernel(const uint32_t* __restr
 }
 
 //! Leaf moments: one unit of one leaf, relative to the leaf's first key and
-//! index. Launched with an upper bound of units; extra blocks return.
+//! index. Launched with an upper bound of units (units_max per candidate);
+//! extra blocks return.
 __global__ __launch_bounds__(256) void leaf_partial_kernel(const uint64_t* __restrict__ s,
-                                                           const uint32_t* __restrict__ lstart,
-                                                           const uint32_t* __restrict__ ustart,
+                                                           const uint32_t* __restrict__ lstart_all,
+                                                           const uint32_t* __restrict__ ustart_all,
+                                                           uint32_t units_max,
                                                            double* __restrict__ part)
 {
+    const uint32_t* lstart = lstart_all + blockIdx.y * (LEAVES + 1);
+    const uint32_t* ustart = ustart_all + blockIdx.y * (LEAVES + 1);
     const uint32_t b = blockIdx.x;
     if (b >= ustart[LEAVES]) return;
     uint32_t lo_l = 0, hi_l = LEAVES; // leaf: last l with ustart[l] <= b
@@ -265,9 +310,10 @@ __global__ __launch_bounds__(256) void leaf_partial_kernel(const uint64_t* __res
     const uint32_t first = lstart[l];
     const uint32_t lo = first + (b - ustart[l]) * FIT_UNIT;
     double mo[NMOM];
-    unit_moments(s, lo, min(lstart[l + 1], lo + FIT_UNIT), s[first], first, mo);
+    unit_moments(s, lo, min(lstart[l + 1], lo + FIT_UNIT), s[first], first, false, mo);
     if (threadIdx.x == 0)
-        for (int k = 0; k < NMOM; ++k) part[size_t(b) * NMOM + k] = mo[k];
+        for (int k = 0; k < NMOM; ++k)
+            part[(size_t(blockIdx.y) * units_max + b) * NMOM + k] = mo[k];
 }
 
 //! Leaf l: linear model of the empirical CDF (slope clamped >= 0) and its
@@ -275,15 +321,19 @@ __global__ __launch_bounds__(256) void leaf_partial_kernel(const uint64_t* __res
 //! range ends where the next begins).
 __global__ __launch_bounds__(256) void leaf_fit_kernel(const uint64_t* __restrict__ s,
                                                        uint32_t m, uint32_t nb,
-                                                       const uint32_t* __restrict__ lstart,
-                                                       const uint32_t* __restrict__ ustart,
+                                                       const uint32_t* __restrict__ lstart_all,
+                                                       const uint32_t* __restrict__ ustart_all,
+                                                       uint32_t units_max,
                                                        const double* __restrict__ part,
-                                                       rmi_model* model)
+                                                       rmi_model* models)
 {
+    const uint32_t c = blockIdx.y;
+    const uint32_t* lstart = lstart_all + c * (LEAVES + 1);
+    const uint32_t* ustart = ustart_all + c * (LEAVES + 1);
     const uint32_t l = blockIdx.x;
     const uint32_t lo = lstart[l], hi = lstart[l + 1];
     double mo[NMOM];
-    combine_moments(part, ustart[l], ustart[l + 1], mo);
+    combine_moments(part + size_t(c) * units_max * NMOM, ustart[l], ustart[l + 1], mo);
     if (threadIdx.x == 0)
     {
         double a = 0, b = (double(lo) + double(hi)) / (2.0 * m);
@@ -297,7 +347,7 @@ __global__ __launch_bounds__(256) void leaf_fit_kernel(const uint64_t* __restric
         const uint32_t blo = static_cast<uint32_t>(uint64_t(nb) * lo / m);
         const uint32_t next =
             l + 1 < LEAVES ? static_cast<uint32_t>(uint64_t(nb) * hi / m) : nb - 1;
-        rmi_leaf& L = model->leaf[l];
+        rmi_leaf& L = models[c].leaf[l];
         L.a = a;
         L.b = b;
         L.lo = min(blo, nb - 1);
@@ -306,14 +356,16 @@ __global__ __launch_bounds__(256) void leaf_fit_kernel(const uint64_t* __restric
 }
 
 //! Per predicted bucket of the sample: its first index (buckets are
-//! non-decreasing along the sorted sample).
+//! non-decreasing along the sorted sample); nb entries per candidate.
 __global__ __launch_bounds__(256) void sample_runs_kernel(const uint64_t* __restrict__ s,
-                                                          uint32_t m, const rmi_model* model,
-                                                          uint32_t* __restrict__ run_start)
+                                                          uint32_t m, uint32_t nb,
+                                                          const rmi_model* models,
+                                                          uint32_t* __restrict__ runs_all)
 {
     __shared__ rmi_model sm;
-    load_model(&sm, model);
+    load_model(&sm, models + blockIdx.y);
     __syncthreads();
+    uint32_t* run_start = runs_all + size_t(blockIdx.y) * nb;
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < m; i += gridDim.x * 256)
     {
         const uint32_t b = bucket_of(sm, s[i]);
@@ -321,26 +373,49 @@ __global__ __launch_bounds__(256) void sample_runs_kernel(const uint64_t* __rest
     }
 }
 
-//! quality[0] = largest predicted bucket (sample keys); quality[1] = sample
-//! keys in buckets whose estimated size exceeds cap and that hold more than
-//! one distinct key (those go through the onesweep fallback).
+//! Per candidate c: quality[2c] = largest predicted bucket (sample keys);
+//! quality[2c + 1] = sample keys in buckets whose estimated size exceeds cap
+//! and that hold more than one distinct key (those would go through the
+//! onesweep fallback). Reduced per block, one atomic per block and value.
 __global__ __launch_bounds__(256) void sample_quality_kernel(
-    const uint64_t* __restrict__ s, uint32_t m, const rmi_model* model,
-    const uint32_t* __restrict__ run_start, double keys_per_sample, uint32_t cap,
+    const uint64_t* __restrict__ s, uint32_t m, uint32_t nb, const rmi_model* models,
+    const uint32_t* __restrict__ runs_all, double keys_per_sample, uint32_t cap,
     unsigned long long* quality)
 {
     __shared__ rmi_model sm;
-    load_model(&sm, model);
+    __shared__ unsigned long long s_max[4], s_sum[4];
+    load_model(&sm, models + blockIdx.y);
     __syncthreads();
+    const uint32_t* run_start = runs_all + size_t(blockIdx.y) * nb;
+    unsigned long long mx = 0, sum = 0;
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < m; i += gridDim.x * 256)
     {
         const uint32_t b = bucket_of(sm, s[i]);
         if (i + 1 < m && bucket_of(sm, s[i + 1]) == b) continue; // not a run end
         const uint32_t first = run_start[b];
         const uint32_t cnt = i + 1 - first;
-        atomicMax(&quality[0], static_cast<unsigned long long>(cnt));
-        if (s[first] != s[i] && double(cnt) * keys_per_sample > double(cap))
-            atomicAdd(&quality[1], static_cast<unsigned long long>(cnt));
+        mx = max(mx, static_cast<unsigned long long>(cnt));
+        if (s[first] != s[i] && double(cnt) * keys_per_sample > double(cap)) sum += cnt;
+    }
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1)
+    {
+        mx = max(mx, static_cast<unsigned long long>(__shfl_xor(mx, o, WAVE)));
+        sum += __shfl_xor(sum, o, WAVE);
+    }
+    const uint32_t lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+    if (lane == 0)
+    {
+        s_max[wave] = mx;
+        s_sum[wave] = sum;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        mx = max(max(s_max[0], s_max[1]), max(s_max[2], s_max[3]));
+        sum = s_sum[0] + s_sum[1] + s_sum[2] + s_sum[3];
+        if (mx != 0) atomicMax(&quality[2 * blockIdx.y], mx);
+        if (sum != 0) atomicAdd(&quality[2 * blockIdx.y + 1], sum);
     }
 }
 
@@ -360,11 +435,22 @@ __global__ __launch_bounds__(256) void id_histogram_kernel(const K* __restrict__
     for (uint32_t i = threadIdx.x; i < 2 * RADIX * C; i += 256) s_h[i] = 0;
     __syncthreads();
     const uint32_t copy = threadIdx.x % C;
-    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+    // U independent loads per thread in flight before any inference.
+    constexpr uint32_t U = 4;
+    const uint32_t stride = gridDim.x * 256;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += U * stride)
     {
-        const uint32_t b = bucket_of(sm, static_cast<uint64_t>(order_bits<K, KIND, DESC>(keys[i])));
-        atomicAdd(&s_h[(b & 0xFFu) * C + copy], 1u);
-        atomicAdd(&s_h[(RADIX + (b >> 8)) * C + copy], 1u);
+        K k[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) k[u] = i + u * stride < n ? keys[i + u * stride] : K(0);
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+        {
+            if (i + u * stride >= n) break;
+            const uint32_t b = bucket_of(sm, static_cast<uint64_t>(order_bits<K, KIND, DESC>(k[u])));
+            atomicAdd(&s_h[(b & 0xFFu) * C + copy], 1u);
+            atomicAdd(&s_h[(RADIX + (b >> 8)) * C + copy], 1u);
+        }
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < 2 * RADIX; i += 256)
@@ -422,8 +508,9 @@ __global__ __launch_bounds__(LBLOCK) void id_onesweep_kernel(
             s_keys, s_vals, s_gbase, s_tmp);
 }
 
-//! Exact bucket offsets of keys already grouped by bucket: off[b] = first
-//! index with bucket >= b, off[nb] = n.
+//! Exact bucket offsets of keys already grouped by non-decreasing bucket:
+//! off[c] = first index whose bucket is >= c, by binary search (one thread
+//! per bucket, about log2(n) dependent reads each), off[nb] = n.
 template <typename K, int KIND, bool DESC>
 __global__ __launch_bounds__(256) void boundary_kernel(const K* __restrict__ keys, uint32_t n,
                                                        const rmi_model* model, uint32_t nb,
@@ -432,16 +519,19 @@ __global__ __launch_bounds__(256) void boundary_kernel(const K* __restrict__ key
     __shared__ rmi_model sm;
     load_model(&sm, model);
     __syncthreads();
-    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+    const uint32_t c = blockIdx.x * 256 + threadIdx.x;
+    if (c > nb) return;
+    uint32_t lo = 0, hi = n; // answer in [lo, hi]
+    if (c == nb) lo = n;
+    while (lo < hi)
     {
-        const uint32_t b = bucket_of(sm, static_cast<uint64_t>(order_bits<K, KIND, DESC>(keys[i])));
-        const int bp = i == 0
-            ? -1
-            : int(bucket_of(sm, static_cast<uint64_t>(order_bits<K, KIND, DESC>(keys[i - 1]))));
-        for (int c = bp + 1; c <= int(b); ++c) off[c] = i;
-        if (i == n - 1)
-            for (uint32_t c = b + 1; c <= nb; ++c) off[c] = n;
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (bucket_of(sm, static_cast<uint64_t>(order_bits<K, KIND, DESC>(keys[mid]))) >= c)
+            hi = mid;
+        else
+            lo = mid + 1;
     }
+    off[c] = lo;
 }
 
 //! stats[0] = largest bucket, stats[1] = keys in buckets above cap.
@@ -490,22 +580,23 @@ void run(context& ctx, K* keys, uint32_t* vals, uint32_t n, int key_type, stats*
     const std::size_t cap = lds_capacity(table);
     const uint32_t tiles = (n + LTILE - 1) / LTILE;
 
+    constexpr uint32_t NC = CANDIDATES;
     layout L;
-    const std::size_t o_quality = L.take(64);
+    const std::size_t o_quality = L.take(NC * 2 * 8);
     const std::size_t o_stats = L.take(64);
     const std::size_t o_hist = L.take(2 * RADIX * 4);
     const std::size_t o_ctr = L.take(64);
-    const std::size_t o_runs = L.take(std::size_t(nb) * 4);
+    const std::size_t o_runs = L.take(std::size_t(NC) * nb * 4);
     const std::size_t o_st0 = L.take(std::size_t(tiles) * RADIX * 4);
     const std::size_t zero_bytes = L.bytes;
     const std::size_t o_st1 = L.take(std::size_t(tiles) * RADIX * 4);
-    const std::size_t o_model = L.take(sizeof(rmi_model));
-    const std::size_t o_lstart = L.take((LEAVES + 1) * 4);
-    const std::size_t o_ustart = L.take((LEAVES + 1) * 4);
+    const std::size_t o_model = L.take(NC * sizeof(rmi_model));
+    const std::size_t o_lstart = L.take(NC * (LEAVES + 1) * 4);
+    const std::size_t o_ustart = L.take(NC * (LEAVES + 1) * 4);
     const uint32_t root_units = (m + FIT_UNIT - 1) / FIT_UNIT;
     const uint32_t leaf_units_max = LEAVES + root_units; // a leaf adds at most one partial unit
-    const std::size_t o_rpart = L.take(std::size_t(root_units) * NMOM * 8);
-    const std::size_t o_lpart = L.take(std::size_t(leaf_units_max) * NMOM * 8);
+    const std::size_t o_rpart = L.take(std::size_t(NC) * root_units * NMOM * 8);
+    const std::size_t o_lpart = L.take(std::size_t(NC) * leaf_units_max * NMOM * 8);
     const std::size_t o_base = L.take(2 * RADIX * 4);
     const std::size_t o_sample = L.take(std::size_t(m) * 8);
     const std::size_t o_off = L.take((std::size_t(nb) + 1) * 8);
@@ -520,7 +611,7 @@ void run(context& ctx, K* keys, uint32_t* vals, uint32_t n, int key_type, stats*
     uint32_t* runs = reinterpret_cast<uint32_t*>(t + o_runs);
     uint32_t* st[2] = { reinterpret_cast<uint32_t*>(t + o_st0),
                         reinterpret_cast<uint32_t*>(t + o_st1) };
-    rmi_model* model = reinterpret_cast<rmi_model*>(t + o_model);
+    rmi_model* models = reinterpret_cast<rmi_model*>(t + o_model);
     uint32_t* lstart = reinterpret_cast<uint32_t*>(t + o_lstart);
     uint32_t* ustart = reinterpret_cast<uint32_t*>(t + o_ustart);
     double* rpart = reinterpret_cast<double*>(t + o_rpart);
@@ -539,32 +630,44 @@ void run(context& ctx, K* keys, uint32_t* vals, uint32_t n, int key_type, stats*
         hipLaunchKernelGGL((sample_kernel<K, KIND, DESC>), dim3(grid), dim3(256), 0, s, keys,
                            m, sample);
         detail::radix_sort(ctx, sample, TY_U64, nullptr, m, false);
-        hipLaunchKernelGGL(root_partial_kernel, dim3(root_units), dim3(256), 0, s, sample, m,
-                           rpart);
-        hipLaunchKernelGGL(root_fit_kernel, dim3(1), dim3(256), 0, s, sample, rpart, root_units,
-                           m, nb, model);
-        hipLaunchKernelGGL(leaf_bounds_kernel, dim3(grid), dim3(256), 0, s, sample, m, model,
-                           lstart);
-        hipLaunchKernelGGL(leaf_units_kernel, dim3(1), dim3(256), 0, s, lstart, ustart);
-        hipLaunchKernelGGL(leaf_partial_kernel, dim3(leaf_units_max), dim3(256), 0, s, sample,
-                           lstart, ustart, lpart);
-        hipLaunchKernelGGL(leaf_fit_kernel, dim3(LEAVES), dim3(256), 0, s, sample, m, nb,
-                           lstart, ustart, lpart, model);
-        hipLaunchKernelGGL(sample_runs_kernel, dim3(grid), dim3(256), 0, s, sample, m, model,
-                           runs);
-        hipLaunchKernelGGL(sample_quality_kernel, dim3(grid), dim3(256), 0, s, sample, m,
-                           model, runs, double(n) / double(m), uint32_t(cap), quality);
+        // Both candidate models at once: blockIdx.y selects the candidate.
+        const uint32_t sgrid = std::max(1u, std::min<uint32_t>((m + 2047) / 2048,
+                                                               uint32_t(ctx.facts().cu_count) * 4));
+        hipLaunchKernelGGL(root_partial_kernel, dim3(root_units, NC), dim3(256), 0, s, sample,
+                           m, root_units, rpart);
+        hipLaunchKernelGGL(root_fit_kernel, dim3(1, NC), dim3(256), 0, s, sample, rpart,
+                           root_units, m, nb, models);
+        hipLaunchKernelGGL(leaf_bounds_kernel, dim3(sgrid, NC), dim3(256), 0, s, sample, m,
+                           models, lstart);
+        hipLaunchKernelGGL(leaf_units_kernel, dim3(1, NC), dim3(256), 0, s, lstart, ustart);
+        hipLaunchKernelGGL(leaf_partial_kernel, dim3(leaf_units_max, NC), dim3(256), 0, s,
+                           sample, lstart, ustart, leaf_units_max, lpart);
+        hipLaunchKernelGGL(leaf_fit_kernel, dim3(LEAVES, NC), dim3(256), 0, s, sample, m, nb,
+                           lstart, ustart, leaf_units_max, lpart, models);
+        hipLaunchKernelGGL(sample_runs_kernel, dim3(sgrid, NC), dim3(256), 0, s, sample, m, nb,
+                           models, runs);
+        hipLaunchKernelGGL(sample_quality_kernel, dim3(sgrid, NC), dim3(256), 0, s, sample, m,
+                           nb, models, runs, double(n) / double(m), uint32_t(cap), quality);
     }
-    unsigned long long q[2] = { 0, 0 };
+    unsigned long long q[2 * NC];
     CLC_HIP(hipMemcpyAsync(ctx.pinned(), quality, sizeof q, hipMemcpyDeviceToHost, s));
     CLC_HIP(hipStreamSynchronize(s));
     std::memcpy(q, ctx.pinned(), sizeof q);
-    const double fit_error = double(q[1]) / double(m);
+    // The candidate with fewer sample keys in overfull buckets, then the
+    // smaller largest bucket; the linear root on a tie.
+    uint32_t pick = 0;
+    for (uint32_t c = 1; c < NC; ++c)
+        if (q[2 * c + 1] < q[2 * pick + 1]
+            || (q[2 * c + 1] == q[2 * pick + 1] && q[2 * c] < q[2 * pick]))
+            pick = c;
+    const rmi_model* model = models + pick;
+    const double fit_error = double(q[2 * pick + 1]) / double(m);
     if (out != nullptr)
     {
         out->num_buckets = nb;
         out->sample_size = m;
         out->sample_fit_error = fit_error;
+        out->root_log = static_cast<int>(pick);
     }
     if (fit_error > FIT_ERROR_LIMIT)
     {
@@ -609,8 +712,8 @@ void run(context& ctx, K* keys, uint32_t* vals, uint32_t n, int key_type, stats*
     }
     {
         timed_launch timer(ctx, "learned_boundaries");
-        hipLaunchKernelGGL((boundary_kernel<K, KIND, DESC>), dim3(grid), dim3(256), 0, s, keys,
-                           n, model, nb, off);
+        hipLaunchKernelGGL((boundary_kernel<K, KIND, DESC>), dim3(nb / 256 + 1), dim3(256), 0,
+                           s, keys, n, model, nb, off);
     }
     detail::segmented_sort(ctx, keys, key_type, vals, n, off, nb, DESC);
     if (out != nullptr)

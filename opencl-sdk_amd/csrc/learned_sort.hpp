@@ -21,6 +21,7 @@ struct stats
     double sample_fit_error = 0;
     double max_bucket_overfill = 0;
     double fallback_fraction = 0;
+    int root_log = 0; //!< 1 when the root is linear in the key's log image
 };
 
 } // namespace learned

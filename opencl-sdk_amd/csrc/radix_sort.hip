@@ -48,22 +48,45 @@ using namespace radix;
 //! Every bin has C = 32 KiB / (P KiB) copies, interleaved (copy = lane mod
 //! C, adjacent copies in adjacent banks): lanes that hit the same digit,
 //! the common case for skewed keys, serialise at most 64 / C deep instead
-//! of 64.
+//! of 64. 64-bit keys (whose high places are often constant: keys below
+//! 2^48, lognormal or Zipf keys) also count the digit of keys[0] at each
+//! place in a register (a constant place's digit; no LDS atomic for it)
+//! and add it once at the end.
 template <typename K, int KIND, bool DESC, int BLOCK, bool VEC>
 __global__ __launch_bounds__(BLOCK) void digit_histogram_kernel(
     const K* __restrict__ keys, uint32_t n, uint32_t* __restrict__ hist)
 {
     constexpr int P = sizeof(K);
     constexpr int C = 32 / P;
+    constexpr bool DOM = P == 8;
     __shared__ uint32_t s_hist[P * RADIX * C];
     for (uint32_t i = threadIdx.x; i < P * RADIX * C; i += BLOCK) s_hist[i] = 0;
-    __syncthreads();
     const uint32_t copy = threadIdx.x % C;
+    uint32_t dom[DOM ? P : 1], hits[DOM ? P : 1];
+    if constexpr (DOM)
+    {
+        const K k0 = keys[0];
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+        {
+            dom[p] = digit_of<K, KIND, DESC>(k0, 8u * p);
+            hits[p] = 0;
+        }
+    }
+    __syncthreads();
     auto count = [&](K k) {
 #pragma unroll
         for (int p = 0; p < P; ++p)
-            atomicAdd(&s_hist[(p * RADIX + digit_of<K, KIND, DESC>(k, 8u * p)) * C + copy],
-                      1u);
+        {
+            const uint32_t d = digit_of<K, KIND, DESC>(k, 8u * p);
+            if constexpr (DOM)
+                if (d == dom[p])
+                {
+                    ++hits[p];
+                    continue;
+                }
+            atomicAdd(&s_hist[(p * RADIX + d) * C + copy], 1u);
+        }
     };
     const uint32_t stride = gridDim.x * BLOCK;
     uint32_t start = 0;
@@ -83,6 +106,14 @@ __global__ __launch_bounds__(BLOCK) void digit_histogram_kernel(
     }
     for (uint32_t i = start + blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride)
         count(keys[i]);
+    if constexpr (DOM)
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+        {
+            const uint32_t h = wave_sum(hits[p]);
+            if (threadIdx.x % WAVE == 0 && h != 0)
+                atomicAdd(&s_hist[(p * RADIX + dom[p]) * C], h);
+        }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < P * RADIX; i += BLOCK)
     {

@@ -31,14 +31,31 @@ __global__ __launch_bounds__(256) void classify_kernel(
     uint32_t large_cap, uint32_t* __restrict__ counts,
     uint32_t* __restrict__ lists, uint32_t list_stride)
 {
-    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < nseg; i += gridDim.x * 256)
+    // The loop bound is uniform per block so that whole waves take part in
+    // the per-class ballots: one atomic per class present in a wave (tens of
+    // thousands of segments on a handful of counters would serialise).
+    for (uint32_t base = blockIdx.x * 256; base < nseg; base += gridDim.x * 256)
     {
-        const uint64_t b = off[i], e = off[i + 1];
-        if (e <= b + 1 || e > num_items) continue;
-        const int c = t.cls[bucket_of(e - b)];
-        const uint32_t pos = atomicAdd(&counts[c], 1u);
-        if (c == LARGE && pos >= large_cap) continue;
-        lists[size_t(c) * list_stride + pos] = i;
+        const uint32_t i = base + threadIdx.x;
+        int c = -1;
+        if (i < nseg)
+        {
+            const uint64_t b = off[i], e = off[i + 1];
+            if (e > b + 1 && e <= num_items) c = t.cls[bucket_of(e - b)];
+        }
+        for (int k = 0; k <= LARGE; ++k)
+        {
+            const uint64_t mask = __ballot(c == k);
+            if (mask == 0) continue;
+            const uint32_t below = count_below(mask);
+            uint32_t first = 0;
+            if (c == k && below == 0) first = atomicAdd(&counts[k], uint32_t(__popcll(mask)));
+            first = __shfl(first, __ffsll(static_cast<unsigned long long>(mask)) - 1, WAVE);
+            if (c != k) continue;
+            const uint32_t pos = first + below;
+            if (k == LARGE && pos >= large_cap) continue;
+            lists[size_t(k) * list_stride + pos] = i;
+        }
     }
 }
 
@@ -142,9 +159,11 @@ __global__ __launch_bounds__(256) void large_hist_kernel(
     uint32_t* __restrict__ hist, uint32_t* queue)
 {
     constexpr int P = sizeof(K);
-    __shared__ uint32_t s_h[P * RADIX];
+    constexpr int C = 32 / P; // interleaved copies per bin, as in the plain histogram
+    __shared__ uint32_t s_h[P * RADIX * C];
     __shared__ uint32_t s_item, s_j;
     const uint32_t L = *large_used, T = *total_tiles;
+    const uint32_t copy = threadIdx.x % C;
     for (;;)
     {
         if (threadIdx.x == 0)
@@ -153,7 +172,7 @@ __global__ __launch_bounds__(256) void large_hist_kernel(
             s_item = t;
             if (t < T) s_j = find_segment(tile_start, L, t);
         }
-        for (uint32_t i = threadIdx.x; i < P * RADIX; i += 256) s_h[i] = 0;
+        for (uint32_t i = threadIdx.x; i < P * RADIX * C; i += 256) s_h[i] = 0;
         __syncthreads();
         const uint32_t t = s_item;
         if (t >= T) break;
@@ -163,16 +182,42 @@ __global__ __launch_bounds__(256) void large_hist_kernel(
         const uint64_t m = off[sidx + 1] - b;
         const uint64_t lo = uint64_t(t - tile_start[j]) * LARGE_TILE;
         const uint64_t hi = std::min<uint64_t>(lo + LARGE_TILE, m);
+        // Digits equal to the tile's first key's (a place constant across
+        // the segment, common for large segments) are counted in registers
+        // and added once: no LDS atomics all on one address.
+        const K k0 = keys[b + lo];
+        uint32_t dom[P], hits[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+        {
+            dom[p] = digit_of<K, KIND, DESC>(k0, 8u * p);
+            hits[p] = 0;
+        }
         for (uint64_t i = lo + threadIdx.x; i < hi; i += 256)
         {
             const K k = keys[b + i];
 #pragma unroll
             for (int p = 0; p < P; ++p)
-                atomicAdd(&s_h[p * RADIX + digit_of<K, KIND, DESC>(k, 8u * p)], 1u);
+            {
+                const uint32_t d = digit_of<K, KIND, DESC>(k, 8u * p);
+                if (d == dom[p]) ++hits[p];
+                else atomicAdd(&s_h[(p * RADIX + d) * C + copy], 1u);
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+        {
+            const uint32_t h = wave_sum(hits[p]);
+            if (threadIdx.x % WAVE == 0 && h != 0) atomicAdd(&s_h[(p * RADIX + dom[p]) * C], h);
         }
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < P * RADIX; i += 256)
-            if (s_h[i] != 0) atomicAdd(&hist[size_t(j) * P * RADIX + i], s_h[i]);
+        {
+            uint32_t c = 0;
+#pragma unroll
+            for (int q = 0; q < C; ++q) c += s_h[i * C + q];
+            if (c != 0) atomicAdd(&hist[size_t(j) * P * RADIX + i], c);
+        }
         __syncthreads();
     }
 }
@@ -180,10 +225,12 @@ __global__ __launch_bounds__(256) void large_hist_kernel(
 //! Persistent: exclusive scan of each (large segment, digit place) count
 //! row, and whether that place must run for that segment: a place where one
 //! digit holds every key of the segment is skipped (identity permutation).
+//! any_exec[p] (zeroed beforehand) is set when some segment runs place p.
 __global__ __launch_bounds__(256) void large_digit_scan_kernel(
     const uint32_t* __restrict__ hist, uint32_t* __restrict__ base,
     const uint32_t* large_used, uint32_t P, const uint64_t* __restrict__ off,
-    const uint32_t* __restrict__ large_list, uint32_t* __restrict__ exec)
+    const uint32_t* __restrict__ large_list, uint32_t* __restrict__ exec,
+    uint32_t* __restrict__ any_exec)
 {
     __shared__ uint32_t tmp[4];
     __shared__ uint32_t s_trivial;
@@ -197,8 +244,76 @@ __global__ __launch_bounds__(256) void large_digit_scan_kernel(
         const uint32_t s = large_list[r / P];
         if (uint64_t(c) == off[s + 1] - off[s]) s_trivial = 1;
         __syncthreads();
-        if (threadIdx.x == 0) exec[r] = s_trivial ? 0u : 1u;
+        if (threadIdx.x == 0)
+        {
+            exec[r] = s_trivial ? 0u : 1u;
+            if (!s_trivial) atomicOr(&any_exec[r % P], 1u);
+        }
         __syncthreads();
+    }
+}
+
+//! Single block: the large segments that run at least one digit place (the
+//! others hold one repeated key and are already in order), as indices into
+//! the large list, with their tile layout: act_start[a] = first tile of
+//! active segment a, act_start[A] = all their tiles. The onesweep passes and
+//! the copy-back walk only these tiles.
+__global__ __launch_bounds__(1024) void active_scan_kernel(
+    const uint64_t* __restrict__ off, const uint32_t* __restrict__ large_list,
+    const uint32_t* large_used, const uint32_t* __restrict__ exec, uint32_t P,
+    uint32_t max_tiles, uint32_t* __restrict__ active, uint32_t* __restrict__ act_start,
+    uint32_t* act_used, uint32_t* act_tiles)
+{
+    __shared__ uint32_t s_f[16], s_t[16];
+    const uint32_t tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
+    const uint32_t L = *large_used;
+    uint32_t carry_f = 0, carry_t = 0;
+    for (uint32_t base = 0; base < L; base += 1024)
+    {
+        const uint32_t j = base + tid;
+        uint32_t f = 0, nt = 0;
+        if (j < L)
+        {
+            for (uint32_t p = 0; p < P; ++p) f |= exec[j * P + p];
+            if (f)
+            {
+                const uint32_t s = large_list[j];
+                nt = static_cast<uint32_t>((off[s + 1] - off[s] + LARGE_TILE - 1) / LARGE_TILE);
+            }
+        }
+        const uint32_t inc_f = wave_inclusive_add(f), inc_t = wave_inclusive_add(nt);
+        if (lane == WAVE - 1)
+        {
+            s_f[wave] = inc_f;
+            s_t[wave] = inc_t;
+        }
+        __syncthreads();
+        uint32_t bf = 0, bt = 0, tf = 0, tt = 0;
+        for (int w = 0; w < 16; ++w)
+        {
+            if (w < static_cast<int>(wave))
+            {
+                bf += s_f[w];
+                bt += s_t[w];
+            }
+            tf += s_f[w];
+            tt += s_t[w];
+        }
+        if (f)
+        {
+            const uint32_t a = carry_f + bf + inc_f - 1;
+            active[a] = j;
+            act_start[a] = carry_t + bt + inc_t - nt;
+        }
+        carry_f += tf;
+        carry_t += tt;
+        __syncthreads();
+    }
+    if (tid == 0)
+    {
+        act_start[carry_f] = carry_t;
+        *act_used = carry_f;
+        *act_tiles = min(carry_t, max_tiles);
     }
 }
 
@@ -208,9 +323,10 @@ template <typename K, bool VALUES>
 __global__ __launch_bounds__(256) void large_copy_back_kernel(
     K* __restrict__ keys, const K* __restrict__ keys_alt, uint32_t* __restrict__ vals,
     const uint32_t* __restrict__ vals_alt, const uint64_t* __restrict__ off,
-    const uint32_t* __restrict__ large_list, const uint32_t* __restrict__ tile_start,
-    const uint32_t* large_used, const uint32_t* total_tiles,
-    const uint32_t* __restrict__ exec, uint32_t P, uint32_t* queue)
+    const uint32_t* __restrict__ large_list, const uint32_t* __restrict__ active,
+    const uint32_t* __restrict__ tile_start, const uint32_t* large_used,
+    const uint32_t* total_tiles, const uint32_t* __restrict__ exec, uint32_t P,
+    uint32_t* queue)
 {
     __shared__ uint32_t s_item, s_j;
     const uint32_t L = *large_used, T = *total_tiles;
@@ -226,10 +342,11 @@ __global__ __launch_bounds__(256) void large_copy_back_kernel(
         const uint32_t t = s_item, j = s_j;
         __syncthreads();
         if (t >= T) break;
+        const uint32_t jf = active[j];
         uint32_t odd = 0;
-        for (uint32_t p = 0; p < P; ++p) odd ^= exec[j * P + p];
+        for (uint32_t p = 0; p < P; ++p) odd ^= exec[jf * P + p];
         if (!odd) continue;
-        const uint32_t sidx = large_list[j];
+        const uint32_t sidx = large_list[jf];
         const uint64_t b = off[sidx];
         const uint64_t m = off[sidx + 1] - b;
         const uint64_t lo = uint64_t(t - tile_start[j]) * LARGE_TILE;
@@ -242,18 +359,21 @@ __global__ __launch_bounds__(256) void large_copy_back_kernel(
     }
 }
 
-//! Persistent onesweep over the tiles of every large segment, one digit
-//! place; look-back chains run within a segment. Per segment, places whose
+//! Persistent onesweep over the tiles of every active large segment (tile
+//! layout and index j from active_scan_kernel; jf = active[j] indexes the
+//! large list), one digit place; look-back chains run within a segment. Per segment, places whose
 //! digit is constant are skipped (exec[j][p] == 0) and each executed place
 //! reads what the previous executed place wrote (a = keys, b = scratch).
+//! A place that no segment runs only clears the next place's status words.
 template <typename K, int KIND, bool DESC, bool VALUES>
 __global__ __launch_bounds__(LARGE_BLOCK) void large_onesweep_kernel(
     K* keys_a, K* keys_b, uint32_t* vals_a, uint32_t* vals_b,
     const uint64_t* __restrict__ off, const uint32_t* __restrict__ large_list,
-    const uint32_t* __restrict__ tile_start, const uint32_t* large_used,
-    const uint32_t* total_tiles, uint32_t pass, const uint32_t* __restrict__ base,
-    const uint32_t* __restrict__ exec, uint32_t* status, uint32_t* status_next,
-    uint32_t* queue)
+    const uint32_t* __restrict__ active, const uint32_t* __restrict__ tile_start,
+    const uint32_t* large_used, const uint32_t* total_tiles, uint32_t pass,
+    const uint32_t* __restrict__ base, const uint32_t* __restrict__ exec,
+    const uint32_t* __restrict__ any_exec,
+    uint32_t* status, uint32_t* status_next, uint32_t* queue)
 {
     constexpr int P = sizeof(K);
     constexpr int W = LARGE_BLOCK / WAVE;
@@ -265,6 +385,14 @@ __global__ __launch_bounds__(LARGE_BLOCK) void large_onesweep_kernel(
     __shared__ uint32_t s_item, s_j;
     const uint32_t tid = threadIdx.x;
     const uint32_t L = *large_used, T = *total_tiles;
+    if (any_exec[pass] == 0)
+    {
+        if (status_next != nullptr)
+            for (size_t i = size_t(blockIdx.x) * LARGE_BLOCK + tid; i < size_t(T) * RADIX;
+                 i += size_t(gridDim.x) * LARGE_BLOCK)
+                status_next[i] = 0;
+        return;
+    }
     for (;;)
     {
         if (tid == 0)
@@ -277,24 +405,24 @@ __global__ __launch_bounds__(LARGE_BLOCK) void large_onesweep_kernel(
         __syncthreads();
         const uint32_t t = s_item;
         if (t >= T) break;
-        const uint32_t j = s_j;
+        const uint32_t j = s_j, jf = active[j];
         if (status_next != nullptr && tid < RADIX) status_next[size_t(t) * RADIX + tid] = 0;
-        if (exec[j * P + pass] == 0)
+        if (exec[jf * P + pass] == 0)
         {
             __syncthreads(); // everyone has read s_item / s_j
             continue;
         }
         uint32_t parity = 0;
-        for (uint32_t p = 0; p < pass; ++p) parity ^= exec[j * P + p];
+        for (uint32_t p = 0; p < pass; ++p) parity ^= exec[jf * P + p];
         const K* keys_in = parity ? keys_b : keys_a;
         K* keys_out = parity ? keys_a : keys_b;
         const uint32_t* vals_in = parity ? vals_b : vals_a;
         uint32_t* vals_out = parity ? vals_a : vals_b;
-        const uint32_t sidx = large_list[j];
+        const uint32_t sidx = large_list[jf];
         const uint64_t b = off[sidx];
         const uint32_t m = static_cast<uint32_t>(off[sidx + 1] - b);
         const uint32_t local = t - tile_start[j];
-        const uint32_t* dbase = base + (size_t(j) * P + pass) * RADIX;
+        const uint32_t* dbase = base + (size_t(jf) * P + pass) * RADIX;
         uint32_t* st = status + size_t(tile_start[j]) * RADIX;
         const uint32_t* vi = VALUES ? vals_in + b : nullptr;
         uint32_t* vo = VALUES ? vals_out + b : nullptr;
@@ -451,6 +579,8 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
     const std::size_t zero_bytes = Lt.bytes;
     const std::size_t o_lists = Lt.take((NCFG + 1) * nseg * 4);
     const std::size_t o_tstart = Lt.take((L_max + 1) * 4);
+    const std::size_t o_active = Lt.take(L_max * 4);
+    const std::size_t o_astart = Lt.take((L_max + 1) * 4);
     const std::size_t o_base = Lt.take(L_max * P * RADIX * 4);
     const std::size_t o_exec = Lt.take(L_max * P * 4);
     const std::size_t o_stB = Lt.take(T_max * RADIX * 4);
@@ -477,17 +607,24 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
     uint32_t* ctr = reinterpret_cast<uint32_t*>(base_ptr + o_ctr);
     // counters: [0, NCFG] list counts, [8, 8 + NCFG) LDS-class queues,
     // 16 histogram queue, 17 tiles in use, 18 large segments in use,
-    // 19 copy-back queue, [24, 24 + P) onesweep queues
+    // 19 copy-back queue, 20 active large segments, 21 their tiles,
+    // [24, 24 + P) onesweep queues, [40, 40 + P) places that some large
+    // segment runs
     uint32_t* counts = ctr;
     uint32_t* lds_q = ctr + 8;
     uint32_t* hist_q = ctr + 16;
     uint32_t* total_tiles = ctr + 17;
     uint32_t* large_used = ctr + 18;
     uint32_t* copy_q = ctr + 19;
+    uint32_t* act_used = ctr + 20;
+    uint32_t* act_tiles = ctr + 21;
     uint32_t* pass_q = ctr + 24;
+    uint32_t* any_exec = ctr + 40;
     uint32_t* exec = reinterpret_cast<uint32_t*>(base_ptr + o_exec);
     uint32_t* lists = reinterpret_cast<uint32_t*>(base_ptr + o_lists);
     uint32_t* tstart = reinterpret_cast<uint32_t*>(base_ptr + o_tstart);
+    uint32_t* active = reinterpret_cast<uint32_t*>(base_ptr + o_active);
+    uint32_t* astart = reinterpret_cast<uint32_t*>(base_ptr + o_astart);
     uint32_t* hist = reinterpret_cast<uint32_t*>(base_ptr + o_hist);
     uint32_t* dbase = reinterpret_cast<uint32_t*>(base_ptr + o_base);
     uint32_t* st[2] = { reinterpret_cast<uint32_t*>(base_ptr + o_stA),
@@ -515,7 +652,10 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
                            large_used, total_tiles, hist, hist_q);
         const uint32_t sgrid = static_cast<uint32_t>(std::min<std::size_t>(L_max * P, 4096));
         hipLaunchKernelGGL(large_digit_scan_kernel, dim3(sgrid), dim3(256), 0, s, hist,
-                           dbase, large_used, uint32_t(P), off, llist, exec);
+                           dbase, large_used, uint32_t(P), off, llist, exec, any_exec);
+        hipLaunchKernelGGL(active_scan_kernel, dim3(1), dim3(1024), 0, s, off, llist,
+                           large_used, exec, uint32_t(P), static_cast<uint32_t>(T_max), active,
+                           astart, act_used, act_tiles);
         K* kb[2] = { keys, reinterpret_cast<K*>(base_ptr + o_kalt) };
         uint32_t* vb[2] = { vals, VALUES ? reinterpret_cast<uint32_t*>(base_ptr + o_valt)
                                          : nullptr };
@@ -523,13 +663,13 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
         {
             timed_launch timer(ctx, "segmented_onesweep");
             hipLaunchKernelGGL(ok, dim3(ogrid), dim3(LARGE_BLOCK), 0, s, kb[0], kb[1],
-                               vb[0], vb[1], off, llist, tstart, large_used, total_tiles,
-                               uint32_t(p), dbase, exec, st[p % 2],
+                               vb[0], vb[1], off, llist, active, astart, act_used,
+                               act_tiles, uint32_t(p), dbase, exec, any_exec, st[p % 2],
                                p + 1 < P ? st[(p + 1) % 2] : nullptr, pass_q + p);
         }
         hipLaunchKernelGGL(ck, dim3(cpgrid), dim3(256), 0, s, kb[0], kb[1], vb[0], vb[1],
-                           off, llist, tstart, large_used, total_tiles, exec, uint32_t(P),
-                           copy_q);
+                           off, llist, active, astart, act_used, act_tiles, exec,
+                           uint32_t(P), copy_q);
     }
     CLC_HIP(hipGetLastError());
     return Lt.bytes;

@@ -4,7 +4,8 @@
 //
 //   classify      each segment goes to a size class by the tuned table
 //                 (bucket = ceil(log2 m)): one of the LDS configurations
-//                 below, or "large"; per-class lists are filled by atomics
+//                 below, or "large"; per-class lists are filled by one
+//                 atomic per class and wave
 //   lds classes   one persistent kernel per configuration; each workgroup
 //                 takes segments from its class list and sorts each one
 //                 wholly in LDS (lds_sort_segment): one read and one write
@@ -12,9 +13,10 @@
 //   large         segments above every LDS capacity: a single-block scan
 //                 lays out their onesweep tiles, one persistent kernel
 //                 counts every digit place per segment, one scans the
-//                 counts, and one persistent onesweep kernel per digit place
-//                 processes all large segments' tiles, with look-back chains
-//                 per segment
+//                 counts, a single-block scan keeps the segments with some
+//                 varying place (one repeated key needs nothing), and one
+//                 persistent onesweep kernel per digit place processes their
+//                 tiles, with look-back chains per segment
 //
 // The reference library has no segmented sort; its per-segment equivalent
 // is one detail::radix_sort call per segment (Algorithm.cpp:453-527).
