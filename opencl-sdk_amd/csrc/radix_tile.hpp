@@ -416,106 +416,181 @@ __device__ __forceinline__ void digit_bases(uint32_t* s_hist, uint32_t* s_tmp)
     __syncthreads();
 }
 
-//! Stable sort of m <= BLOCK * ITEMS keys (and values) by a single block,
-//! entirely in LDS: one pass per 8-bit digit place that is not the same for
-//! every key of the segment (found by an OR/AND reduction of the keys'
-//! order images), each a wave-ballot rank plus an LDS reorder, then one
-//! contiguous write back (none if no place varies). Keys past m take digit
-//! 255 in every pass; they sit at the end of the tile, so stability keeps
-//! them behind every real key.
-template <typename K, int KIND, bool DESC, bool VALUES, int BLOCK, int ITEMS>
-__device__ __forceinline__ void lds_sort_segment(
-    K* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t m,
-    uint32_t* s_hist, K* s_keys, uint32_t* s_vals, uint32_t* s_tmp)
+//! One stable LDS counting pass of a block over words w[] (row r of wave
+//! `wave` holds position wave * 64 * ITEMS + r * 64 + lane) by the digit
+//! (w >> shift) & 0xFF: wave-ballot ranks, per-wave digit counters, an LDS
+//! reorder. Only positions below m are keys; rows of a wave that hold none
+//! are skipped, and the other positions past m take digit 255, so they end
+//! behind every key (stability). With MOVE set, values v[] follow the
+//! words through s_v in the same reorder.
+template <typename W, int MOVE, int BLOCK, int ITEMS>
+__device__ __forceinline__ void lds_pass(W (&w)[ITEMS], uint32_t (&v)[ITEMS], uint32_t shift,
+                                         uint32_t m, uint32_t* s_hist, W* s_w, uint32_t* s_v,
+                                         uint32_t* s_tmp)
 {
-    constexpr int W = BLOCK / WAVE;
-    constexpr int P = sizeof(K);
+    constexpr int NW = BLOCK / WAVE;
     const uint32_t tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
     const uint32_t wave_base = wave * (WAVE * ITEMS);
+    const uint32_t live =
+        m > wave_base ? min(static_cast<uint32_t>(ITEMS), (m - wave_base + WAVE - 1) / WAVE) : 0u;
     uint32_t* my_hist = s_hist + wave * RADIX;
-
-    __shared__ unsigned long long s_or, s_and;
-    if (tid == 0)
-    {
-        s_or = 0;
-        s_and = ~0ull;
-    }
-    K k[ITEMS];
-    uint32_t v[VALUES ? ITEMS : 1];
-    unsigned long long b_or = 0, b_and = ~0ull;
+    for (uint32_t i = tid; i < NW * RADIX; i += BLOCK) s_hist[i] = 0;
+    __syncthreads();
+    uint32_t rank[ITEMS];
+    uint32_t dg[(ITEMS + 3) / 4]; // digits, four to a register
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r)
-    {
-        const uint32_t pos = wave_base + r * WAVE + lane;
-        k[r] = pos < m ? keys[pos] : K(0);
-        if constexpr (VALUES) v[r] = pos < m ? vals[pos] : 0u;
-        if (pos < m)
-        {
-            const unsigned long long o = order_bits<K, KIND, DESC>(k[r]);
-            b_or |= o;
-            b_and &= o;
-        }
-    }
-    // Digit places where every key of the segment agrees (OR == AND on the
-    // byte) would leave the order unchanged: they are skipped.
-#pragma unroll
-    for (int o = WAVE / 2; o > 0; o >>= 1)
-    {
-        b_or |= __shfl_xor(b_or, o, WAVE);
-        b_and &= __shfl_xor(b_and, o, WAVE);
-    }
-    __syncthreads(); // s_or / s_and initialised
-    if (lane == 0)
-    {
-        atomicOr(&s_or, b_or);
-        atomicAnd(&s_and, b_and);
-    }
-    __syncthreads();
-    const unsigned long long varying = s_or ^ s_and;
-    bool sorted_any = false;
-#pragma unroll 1
-    for (int p = 0; p < P; ++p)
-    {
-        const uint32_t shift = 8u * p;
-        if (((varying >> shift) & 0xFFu) == 0) continue; // uniform across the block
-        sorted_any = true;
-        for (uint32_t i = tid; i < W * RADIX; i += BLOCK) s_hist[i] = 0;
-        __syncthreads();
-        uint32_t rank[ITEMS];
-        uint32_t dig[ITEMS];
-#pragma unroll
-        for (int r = 0; r < ITEMS; ++r)
+        if (static_cast<uint32_t>(r) < live)
         {
             const bool valid = wave_base + r * WAVE + lane < m;
-            const uint32_t d = valid ? digit_of<K, KIND, DESC>(k[r], shift) : RADIX - 1;
-            dig[r] = d;
+            const uint32_t d = valid ? static_cast<uint32_t>(w[r] >> shift) & 0xFFu : RADIX - 1;
+            dg[r / 4] = r % 4 == 0 ? d : dg[r / 4] | (d << (8 * (r % 4)));
             const uint64_t peers = match_digit(d);
             const uint32_t pre = count_below(peers);
             const uint32_t base = my_hist[d];
             if (pre == 0) my_hist[d] = base + __popcll(peers);
             rank[r] = base + pre;
         }
-        __syncthreads();
-        digit_bases<BLOCK>(s_hist, s_tmp);
+    __syncthreads();
+    digit_bases<BLOCK>(s_hist, s_tmp);
 #pragma unroll
-        for (int r = 0; r < ITEMS; ++r)
+    for (int r = 0; r < ITEMS; ++r)
+        if (static_cast<uint32_t>(r) < live)
         {
-            const uint32_t slot = my_hist[dig[r]] + rank[r];
-            s_keys[slot] = k[r];
-            if constexpr (VALUES) s_vals[slot] = v[r];
+            rank[r] += my_hist[(dg[r / 4] >> (8 * (r % 4))) & 0xFFu]; // the slot
+            s_w[rank[r]] = w[r];
+            if constexpr (MOVE == 1) s_v[rank[r]] = v[r];
         }
-        __syncthreads();
+    __syncthreads();
 #pragma unroll
-        for (int r = 0; r < ITEMS; ++r)
+    for (int r = 0; r < ITEMS; ++r)
+        if (static_cast<uint32_t>(r) < live)
         {
-            const uint32_t pos = wave_base + r * WAVE + lane;
-            k[r] = s_keys[pos];
-            if constexpr (VALUES) v[r] = s_vals[pos];
+            w[r] = s_w[wave_base + r * WAVE + lane];
+            if constexpr (MOVE == 1) v[r] = s_v[wave_base + r * WAVE + lane];
         }
-        __syncthreads();
-    }
-    if (sorted_any)
+    __syncthreads();
+}
+
+//! LDS bytes per key of lds_sort_segment's staging buffer: 64-bit keys
+//! with values stage packed words (values move once, outside LDS), 32-bit
+//! keys stage values beside them.
+constexpr std::size_t lds_segment_bytes_per_key(std::size_t kbytes, bool values)
+{
+    return kbytes == 8 ? 8 : 4 + (values ? 4 : 0);
+}
+
+//! Stable sort of m <= BLOCK * ITEMS keys (and values) by a single block,
+//! entirely in LDS, over s_buf (BLOCK * ITEMS * lds_segment_bytes_per_key
+//! bytes). The keys' order images are taken relative to their minimum:
+//! only ceil(log2(max - min + 1) / 8) digit places can differ, one pass
+//! each (none for one repeated key, which is left as it is). 64-bit keys
+//! with values are sorted as packed words, (digits << 16) | original
+//! position, 48 bits of the relative image at a time: the passes move 8
+//! bytes a key and no value, and each value is fetched once at the end by
+//! position (and each key too when the range passes 48 bits). Then one
+//! contiguous write back.
+template <typename K, int KIND, bool DESC, bool VALUES, int BLOCK, int ITEMS>
+__device__ __forceinline__ void lds_sort_segment(
+    K* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t m,
+    uint32_t* s_hist, void* s_buf, uint32_t* s_tmp)
+{
+    constexpr bool WIDE = sizeof(K) == 8;
+    constexpr uint32_t TILE = BLOCK * ITEMS;
+    static_assert(TILE <= 65536, "positions are packed in 16 bits");
+    const uint32_t tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
+    const uint32_t wave_base = wave * (WAVE * ITEMS);
+
+    using U = typename std::conditional<WIDE, unsigned long long, unsigned int>::type;
+    __shared__ U s_min, s_max;
+    if (tid == 0)
     {
+        s_min = ~U(0);
+        s_max = 0;
+    }
+    K k[ITEMS];
+    U lo = ~U(0), hi = 0;
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r)
+    {
+        const uint32_t pos = wave_base + r * WAVE + lane;
+        const bool valid = pos < m;
+        k[r] = valid ? order_bits<K, KIND, DESC>(keys[pos]) : K(0);
+        if (valid)
+        {
+            lo = min(lo, static_cast<U>(k[r]));
+            hi = max(hi, static_cast<U>(k[r]));
+        }
+    }
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1)
+    {
+        lo = min(lo, static_cast<U>(__shfl_xor(lo, o, WAVE)));
+        hi = max(hi, static_cast<U>(__shfl_xor(hi, o, WAVE)));
+    }
+    __syncthreads(); // s_min / s_max initialised
+    if (lane == 0 && wave_base < m)
+    {
+        atomicMin(&s_min, lo);
+        atomicMax(&s_max, hi);
+    }
+    __syncthreads();
+    const K kmin = static_cast<K>(s_min), range = static_cast<K>(s_max - s_min);
+    if (range == 0)
+    {
+        __syncthreads(); // everyone has read s_min / s_max
+        return;
+    }
+    const uint32_t bits = WIDE ? 64u - __clzll(static_cast<unsigned long long>(range))
+                               : 32u - __clz(static_cast<uint32_t>(range));
+    const uint32_t passes = (bits + 7) / 8;
+
+    if constexpr (WIDE && VALUES)
+    {
+        // Words (digits << 16) | original position. The relative image's
+        // low 48 bits are sorted first; a wider range then re-packs each
+        // word with the image's high 16 bits, fetched by position from the
+        // (not yet overwritten) input, and sorts those.
+        uint64_t* s_w = static_cast<uint64_t*>(s_buf);
+        uint32_t unused[ITEMS];
+#pragma unroll
+        for (int r = 0; r < ITEMS; ++r)
+            k[r] = (static_cast<uint64_t>(k[r] - kmin) << 16) | (wave_base + r * WAVE + lane);
+        const uint32_t low_passes = min(passes, 6u);
+#pragma unroll 1
+        for (uint32_t q = 0; q < low_passes; ++q)
+            lds_pass<uint64_t, 0, BLOCK, ITEMS>(k, unused, 16 + 8 * q, m, s_hist, s_w, nullptr,
+                                                s_tmp);
+        if (passes > 6)
+        {
+#pragma unroll
+            for (int r = 0; r < ITEMS; ++r)
+                if (wave_base + r * WAVE + lane < m)
+                {
+                    const uint32_t at = static_cast<uint32_t>(k[r]) & 0xFFFFu;
+                    const uint64_t rel = order_bits<K, KIND, DESC>(keys[at]) - kmin;
+                    k[r] = ((rel >> 48) << 16) | at;
+                }
+#pragma unroll 1
+            for (uint32_t q = 6; q < passes; ++q)
+                lds_pass<uint64_t, 0, BLOCK, ITEMS>(k, unused, 16 + 8 * (q - 6), m, s_hist, s_w,
+                                                    nullptr, s_tmp);
+        }
+        // Each value (and, for a wide range, each key) by its original
+        // position; all are read before any is overwritten in place (the
+        // wait and the barrier).
+        uint32_t v[ITEMS];
+#pragma unroll
+        for (int r = 0; r < ITEMS; ++r)
+            if (wave_base + r * WAVE + lane < m)
+            {
+                const uint32_t at = static_cast<uint32_t>(k[r]) & 0xFFFFu;
+                v[r] = vals[at];
+                if (passes > 6) k[r] = keys[at];
+                else k[r] = from_order_bits<K, KIND, DESC>(static_cast<K>(kmin + (k[r] >> 16)));
+            }
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
 #pragma unroll
         for (int r = 0; r < ITEMS; ++r)
         {
@@ -523,8 +598,34 @@ __device__ __forceinline__ void lds_sort_segment(
             if (pos < m)
             {
                 keys[pos] = k[r];
-                if constexpr (VALUES) vals[pos] = v[r];
+                vals[pos] = v[r];
             }
+        }
+        __syncthreads();
+        return;
+    }
+    constexpr int MOVE = VALUES ? 1 : 0; // 32-bit keys: values beside them
+    K* s_w = static_cast<K*>(s_buf);
+    uint32_t* s_v = reinterpret_cast<uint32_t*>(s_w + TILE);
+    uint32_t v[ITEMS];
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r)
+    {
+        k[r] -= kmin;
+        const uint32_t pos = wave_base + r * WAVE + lane;
+        v[r] = VALUES && pos < m ? vals[pos] : 0u;
+    }
+#pragma unroll 1
+    for (uint32_t q = 0; q < passes; ++q)
+        lds_pass<K, MOVE, BLOCK, ITEMS>(k, v, 8 * q, m, s_hist, s_w, s_v, s_tmp);
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r)
+    {
+        const uint32_t pos = wave_base + r * WAVE + lane;
+        if (pos < m)
+        {
+            keys[pos] = from_order_bits<K, KIND, DESC>(static_cast<K>(k[r] + kmin));
+            if constexpr (VALUES) vals[pos] = v[r];
         }
     }
     __syncthreads();

@@ -67,11 +67,9 @@ __global__ __launch_bounds__(BLOCK) void lds_class_kernel(
     const uint32_t* __restrict__ list, const uint32_t* count, uint32_t* queue)
 {
     constexpr int W = BLOCK / WAVE;
-    constexpr uint32_t TILE = BLOCK * ITEMS;
     extern __shared__ __align__(16) unsigned char smem[];
     uint32_t* s_hist = reinterpret_cast<uint32_t*>(smem);
-    K* s_keys = reinterpret_cast<K*>(smem + W * 1024);
-    uint32_t* s_vals = reinterpret_cast<uint32_t*>(smem + W * 1024 + TILE * sizeof(K));
+    void* s_buf = smem + W * 1024; // TILE * lds_segment_bytes_per_key bytes
     __shared__ uint32_t s_tmp[4];
     __shared__ uint32_t s_item;
     const uint32_t cnt = *count;
@@ -86,7 +84,7 @@ __global__ __launch_bounds__(BLOCK) void lds_class_kernel(
         const uint64_t b = off[sidx];
         const uint32_t m = static_cast<uint32_t>(off[sidx + 1] - b);
         lds_sort_segment<K, KIND, DESC, VALUES, BLOCK, ITEMS>(
-            keys + b, VALUES ? vals + b : nullptr, m, s_hist, s_keys, s_vals, s_tmp);
+            keys + b, VALUES ? vals + b : nullptr, m, s_hist, s_buf, s_tmp);
     }
 }
 
@@ -530,14 +528,7 @@ void launch_lds(context& ctx, bool prepare, int c, K* keys, uint32_t* vals,
         CLC_LDS_CASE(3)
         CLC_LDS_CASE(4)
         CLC_LDS_CASE(5)
-        case 6:
-            if constexpr (sizeof(K) == 4)
-            {
-                launch_lds_class<K, KIND, DESC, VALUES, 6>(ctx, prepare, keys, vals,
-                                                           off, list, count, queue);
-                return;
-            }
-            break;
+        CLC_LDS_CASE(6)
     }
 #undef CLC_LDS_CASE
     throw std::invalid_argument("segmented sort: configuration not available for this key type");

@@ -26,7 +26,6 @@ table default_table(std::size_t kbytes, bool values, std::size_t lds_limit)
         for (int c = 0; c < NCFG; ++c)
         {
             if ((std::size_t(1) << b) > cfg_capacity(c)) continue;
-            if (kbytes == 8 && c == NCFG - 1) continue; // no 64-bit instantiation
             if (cfg_lds(c, kbytes, values) > lds_limit) continue;
             t.cls[b] = c;
             break;
@@ -89,7 +88,6 @@ bool load_table(const context& ctx, int key_type, bool values, table& t)
         if (r.cls[b] < 0 || r.cls[b] > LARGE) return false;
         if (r.cls[b] < NCFG
             && (cfg_lds(r.cls[b], key_bytes(key_type), values) > lds_limit(ctx)
-                || (key_bytes(key_type) == 8 && r.cls[b] == NCFG - 1)
                 || (b > 0 && (std::size_t(1) << b) > cfg_capacity(r.cls[b]))))
             return false;
     }
@@ -175,8 +173,7 @@ table run_tuner(context& ctx, int key_type, bool values)
                                hipMemcpyHostToDevice, s));
         std::vector<int> cands;
         for (int c = 0; c < NCFG; ++c)
-            if (cfg_capacity(c) >= (std::size_t(1) << b) && cfg_lds(c, kb, values) <= limit
-                && !(kb == 8 && c == NCFG - 1))
+            if (cfg_capacity(c) >= (std::size_t(1) << b) && cfg_lds(c, kb, values) <= limit)
                 cands.push_back(c);
         if (b >= 12) cands.push_back(LARGE);
         float best_ms = 1e30f;

@@ -56,8 +56,8 @@ inline std::size_t cfg_capacity(int c)
 inline std::size_t cfg_lds(int c, std::size_t kbytes, bool values)
 {
     const std::size_t tile = cfg_capacity(c);
-    return std::size_t(CFG_BLOCK[c] / 64) * 1024 + tile * kbytes
-        + (values ? tile * 4 : 0) + 64;
+    return std::size_t(CFG_BLOCK[c] / 64) * 1024
+        + tile * radix::lds_segment_bytes_per_key(kbytes, values) + 64;
 }
 
 //! Class per bucket; the autotuner fills it, defaults come from

@@ -60,6 +60,19 @@ def test_class_boundaries_u32_pairs(ctx):
     check(ctx, keys, vals, off)
 
 
+@pytest.mark.parametrize("bits", [16, 48, 49, 64])
+def test_class_boundaries_u64_pairs(ctx, bits):
+    """64-bit keys with values are sorted in LDS as packed (digits << 16 |
+    position) words, 48 bits of the key range at a time: ranges of up to
+    48 bits and just past it take different paths."""
+    say("segmented class boundaries u64 pairs", bits)
+    off = boundary_offsets()
+    n = int(off[-1])
+    keys = oracle.pcg32_u64(5, bits, n, bits=bits)
+    keys[::5] = keys[3]  # equal keys: stability
+    check(ctx, keys, np.arange(n, dtype=np.uint32), off)
+
+
 @pytest.mark.parametrize("dtype", [np.uint32, np.int32, np.float32,
                                    np.uint64, np.int64, np.float64])
 @pytest.mark.parametrize("descending", [False, True])
