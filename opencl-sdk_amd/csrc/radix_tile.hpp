@@ -489,7 +489,8 @@ constexpr std::size_t lds_segment_bytes_per_key(std::size_t kbytes, bool values)
 //! position, 48 bits of the relative image at a time: the passes move 8
 //! bytes a key and no value, and each value is fetched once at the end by
 //! position (and each key too when the range passes 48 bits). Then one
-//! contiguous write back.
+//! contiguous write back. LDS (s_hist, s_buf, s_tmp) may be reused after
+//! the caller's next barrier.
 template <typename K, int KIND, bool DESC, bool VALUES, int BLOCK, int ITEMS>
 __device__ __forceinline__ void lds_sort_segment(
     K* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t m,
@@ -508,15 +509,30 @@ __device__ __forceinline__ void lds_sort_segment(
         s_min = ~U(0);
         s_max = 0;
     }
+    const uint32_t live =
+        m > wave_base ? min(static_cast<uint32_t>(ITEMS), (m - wave_base + WAVE - 1) / WAVE) : 0u;
+    // Loads first, unconditionally (positions past m read key m - 1), so
+    // they are all in flight together: no load waits on a lane or row test
+    // (row tests here also cost the compiler registers).
+    // 32-bit keys load their values with them (one round trip); 64-bit
+    // keys fetch values once, by position, after sorting.
+    constexpr bool EARLY_VALUES = VALUES && !WIDE;
     K k[ITEMS];
+    uint32_t v[ITEMS];
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r)
+    {
+        const uint32_t at = min(wave_base + r * WAVE + lane, m - 1);
+        k[r] = keys[at];
+        v[r] = 0;
+        if constexpr (EARLY_VALUES) v[r] = vals[at];
+    }
     U lo = ~U(0), hi = 0;
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r)
     {
-        const uint32_t pos = wave_base + r * WAVE + lane;
-        const bool valid = pos < m;
-        k[r] = valid ? order_bits<K, KIND, DESC>(keys[pos]) : K(0);
-        if (valid)
+        k[r] = order_bits<K, KIND, DESC>(k[r]);
+        if (wave_base + r * WAVE + lane < m)
         {
             lo = min(lo, static_cast<U>(k[r]));
             hi = max(hi, static_cast<U>(k[r]));
@@ -563,13 +579,22 @@ __device__ __forceinline__ void lds_sort_segment(
                                                 s_tmp);
         if (passes > 6)
         {
+            // Positions past m fetch position 0 (in bounds, never used).
+            K hi16[ITEMS];
 #pragma unroll
             for (int r = 0; r < ITEMS; ++r)
-                if (wave_base + r * WAVE + lane < m)
+                if (static_cast<uint32_t>(r) < live)
                 {
-                    const uint32_t at = static_cast<uint32_t>(k[r]) & 0xFFFFu;
-                    const uint64_t rel = order_bits<K, KIND, DESC>(keys[at]) - kmin;
-                    k[r] = ((rel >> 48) << 16) | at;
+                    const uint32_t at = wave_base + r * WAVE + lane < m
+                        ? static_cast<uint32_t>(k[r]) & 0xFFFFu : 0u;
+                    hi16[r] = keys[at];
+                }
+#pragma unroll
+            for (int r = 0; r < ITEMS; ++r)
+                if (static_cast<uint32_t>(r) < live)
+                {
+                    const uint64_t rel = order_bits<K, KIND, DESC>(hi16[r]) - kmin;
+                    k[r] = ((rel >> 48) << 16) | (k[r] & 0xFFFFu);
                 }
 #pragma unroll 1
             for (uint32_t q = 6; q < passes; ++q)
@@ -579,12 +604,12 @@ __device__ __forceinline__ void lds_sort_segment(
         // Each value (and, for a wide range, each key) by its original
         // position; all are read before any is overwritten in place (the
         // wait and the barrier).
-        uint32_t v[ITEMS];
 #pragma unroll
         for (int r = 0; r < ITEMS; ++r)
-            if (wave_base + r * WAVE + lane < m)
+            if (static_cast<uint32_t>(r) < live)
             {
-                const uint32_t at = static_cast<uint32_t>(k[r]) & 0xFFFFu;
+                const uint32_t at = wave_base + r * WAVE + lane < m
+                    ? static_cast<uint32_t>(k[r]) & 0xFFFFu : 0u;
                 v[r] = vals[at];
                 if (passes > 6) k[r] = keys[at];
                 else k[r] = from_order_bits<K, KIND, DESC>(static_cast<K>(kmin + (k[r] >> 16)));
@@ -601,20 +626,14 @@ __device__ __forceinline__ void lds_sort_segment(
                 vals[pos] = v[r];
             }
         }
-        __syncthreads();
-        return;
+        return; // no LDS use after the last pass's barrier
+
     }
     constexpr int MOVE = VALUES ? 1 : 0; // 32-bit keys: values beside them
     K* s_w = static_cast<K*>(s_buf);
     uint32_t* s_v = reinterpret_cast<uint32_t*>(s_w + TILE);
-    uint32_t v[ITEMS];
 #pragma unroll
-    for (int r = 0; r < ITEMS; ++r)
-    {
-        k[r] -= kmin;
-        const uint32_t pos = wave_base + r * WAVE + lane;
-        v[r] = VALUES && pos < m ? vals[pos] : 0u;
-    }
+    for (int r = 0; r < ITEMS; ++r) k[r] -= kmin;
 #pragma unroll 1
     for (uint32_t q = 0; q < passes; ++q)
         lds_pass<K, MOVE, BLOCK, ITEMS>(k, v, 8 * q, m, s_hist, s_w, s_v, s_tmp);
@@ -628,7 +647,6 @@ __device__ __forceinline__ void lds_sort_segment(
             if constexpr (VALUES) vals[pos] = v[r];
         }
     }
-    __syncthreads();
 }
 
 } // namespace radix

@@ -22,14 +22,16 @@ __device__ __forceinline__ int bucket_of(uint64_t m)
     return b < NBUCKET ? b : NBUCKET - 1;
 }
 
-//! Files every segment of >= 2 keys under its class. Segments that break
-//! the contract (end < begin, end > num_items) are left untouched, and the
-//! large class never takes more than large_cap segments, so malformed
-//! offsets cannot drive any kernel out of bounds.
+//! Files every segment of >= 2 keys under its class: an LDS class gets the
+//! entry (begin low, begin high, size, 0) in its list (stride nseg), the
+//! large class the segment's index. Segments that break the contract (end
+//! < begin, end > num_items) are left untouched, and the large class never
+//! takes more than large_cap segments, so malformed offsets cannot drive
+//! any kernel out of bounds.
 __global__ __launch_bounds__(256) void classify_kernel(
     const uint64_t* __restrict__ off, uint32_t nseg, uint64_t num_items, table t,
-    uint32_t large_cap, uint32_t* __restrict__ counts,
-    uint32_t* __restrict__ lists, uint32_t list_stride)
+    uint32_t large_cap, uint32_t* __restrict__ counts, uint4* __restrict__ entries,
+    uint32_t* __restrict__ large_list)
 {
     // The loop bound is uniform per block so that whole waves take part in
     // the per-class ballots: one atomic per class present in a wave (tens of
@@ -38,9 +40,11 @@ __global__ __launch_bounds__(256) void classify_kernel(
     {
         const uint32_t i = base + threadIdx.x;
         int c = -1;
+        uint64_t b = 0, e = 0;
         if (i < nseg)
         {
-            const uint64_t b = off[i], e = off[i + 1];
+            b = off[i];
+            e = off[i + 1];
             if (e > b + 1 && e <= num_items) c = t.cls[bucket_of(e - b)];
         }
         for (int k = 0; k <= LARGE; ++k)
@@ -53,38 +57,57 @@ __global__ __launch_bounds__(256) void classify_kernel(
             first = __shfl(first, __ffsll(static_cast<unsigned long long>(mask)) - 1, WAVE);
             if (c != k) continue;
             const uint32_t pos = first + below;
-            if (k == LARGE && pos >= large_cap) continue;
-            lists[size_t(k) * list_stride + pos] = i;
+            if (k == LARGE)
+            {
+                if (pos < large_cap) large_list[pos] = i;
+            }
+            else
+                entries[size_t(k) * nseg + pos] =
+                    make_uint4(static_cast<uint32_t>(b), static_cast<uint32_t>(b >> 32),
+                               static_cast<uint32_t>(e - b), 0u);
         }
     }
 }
 
-//! Persistent: each workgroup takes segments of its class and sorts each
-//! wholly in LDS.
+//! Persistent: each workgroup takes segments of its class (entries from
+//! classify_kernel) and sorts each wholly in LDS. Thread 0 claims the next
+//! queue slot while the current segment is sorted, so only the read of
+//! the next entry is exposed between segments.
 template <typename K, int KIND, bool DESC, bool VALUES, int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void lds_class_kernel(
-    K* keys, uint32_t* vals, const uint64_t* __restrict__ off,
-    const uint32_t* __restrict__ list, const uint32_t* count, uint32_t* queue)
+    K* keys, uint32_t* vals, const uint4* __restrict__ entries, const uint32_t* count,
+    uint32_t* queue)
 {
     constexpr int W = BLOCK / WAVE;
     extern __shared__ __align__(16) unsigned char smem[];
     uint32_t* s_hist = reinterpret_cast<uint32_t*>(smem);
     void* s_buf = smem + W * 1024; // TILE * lds_segment_bytes_per_key bytes
     __shared__ uint32_t s_tmp[4];
-    __shared__ uint32_t s_item;
+    __shared__ uint4 s_entry;
+    __shared__ uint32_t s_ok;
     const uint32_t cnt = *count;
+    uint32_t next = 0; // thread 0: the claimed queue slot
+    if (threadIdx.x == 0) next = atomicAdd(queue, 1u);
     for (;;)
     {
-        if (threadIdx.x == 0) s_item = atomicAdd(queue, 1u);
+        if (threadIdx.x == 0)
+        {
+            const bool ok = next < cnt;
+            s_ok = ok;
+            if (ok)
+            {
+                s_entry = entries[next];
+                next = atomicAdd(queue, 1u);
+            }
+        }
         __syncthreads();
-        const uint32_t it = s_item;
-        __syncthreads();
-        if (it >= cnt) break;
-        const uint32_t sidx = list[it];
-        const uint64_t b = off[sidx];
-        const uint32_t m = static_cast<uint32_t>(off[sidx + 1] - b);
+        const uint4 e = s_entry;
+        const bool ok = s_ok;
+        __syncthreads(); // s_entry / s_ok are rewritten next round
+        if (!ok) break;
+        const uint64_t b = e.x | (static_cast<uint64_t>(e.y) << 32);
         lds_sort_segment<K, KIND, DESC, VALUES, BLOCK, ITEMS>(
-            keys + b, VALUES ? vals + b : nullptr, m, s_hist, s_buf, s_tmp);
+            keys + b, VALUES ? vals + b : nullptr, e.z, s_hist, s_buf, s_tmp);
     }
 }
 
@@ -498,8 +521,7 @@ uint32_t lds_class_grid(const context& ctx)
 
 template <typename K, int KIND, bool DESC, bool VALUES, int C>
 void launch_lds_class(context& ctx, bool prepare, K* keys, uint32_t* vals,
-                      const uint64_t* off, const uint32_t* list, const uint32_t* count,
-                      uint32_t* queue)
+                      const uint4* entries, const uint32_t* count, uint32_t* queue)
 {
     constexpr int BLOCK = CFG_BLOCK[C], ITEMS = CFG_ITEMS[C];
     const uint32_t grid = lds_class_grid<K, KIND, DESC, VALUES, C>(ctx);
@@ -507,18 +529,17 @@ void launch_lds_class(context& ctx, bool prepare, K* keys, uint32_t* vals,
     timed_launch timer(ctx, "segmented_lds_sort");
     hipLaunchKernelGGL((lds_class_kernel<K, KIND, DESC, VALUES, BLOCK, ITEMS>), dim3(grid),
                        dim3(BLOCK), cfg_lds(C, sizeof(K), VALUES), ctx.stream(), keys,
-                       vals, off, list, count, queue);
+                       vals, entries, count, queue);
 }
 
 template <typename K, int KIND, bool DESC, bool VALUES>
 void launch_lds(context& ctx, bool prepare, int c, K* keys, uint32_t* vals,
-                const uint64_t* off, const uint32_t* list, const uint32_t* count,
-                uint32_t* queue)
+                const uint4* entries, const uint32_t* count, uint32_t* queue)
 {
 #define CLC_LDS_CASE(C)                                                        \
     case C:                                                                    \
         launch_lds_class<K, KIND, DESC, VALUES, C>(ctx, prepare, keys, vals,   \
-                                                   off, list, count, queue);   \
+                                                   entries, count, queue);     \
         return;
     switch (c)
     {
@@ -568,7 +589,8 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
     const std::size_t o_stA = Lt.take(T_max * RADIX * 4);
     const std::size_t o_hist = Lt.take(L_max * P * RADIX * 4);
     const std::size_t zero_bytes = Lt.bytes;
-    const std::size_t o_lists = Lt.take((NCFG + 1) * nseg * 4);
+    const std::size_t o_entries = Lt.take(std::size_t(NCFG) * nseg * 16);
+    const std::size_t o_llist = Lt.take(nseg * 4);
     const std::size_t o_tstart = Lt.take((L_max + 1) * 4);
     const std::size_t o_active = Lt.take(L_max * 4);
     const std::size_t o_astart = Lt.take((L_max + 1) * 4);
@@ -588,8 +610,8 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
     {
         for (int c = 0; c < NCFG; ++c)
             if (used[c])
-                launch_lds<K, KIND, DESC, VALUES>(ctx, true, c, nullptr, nullptr,
-                                                  nullptr, nullptr, nullptr, nullptr);
+                launch_lds<K, KIND, DESC, VALUES>(ctx, true, c, nullptr, nullptr, nullptr,
+                                                  nullptr, nullptr);
         return Lt.bytes;
     }
 
@@ -612,7 +634,8 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
     uint32_t* pass_q = ctr + 24;
     uint32_t* any_exec = ctr + 40;
     uint32_t* exec = reinterpret_cast<uint32_t*>(base_ptr + o_exec);
-    uint32_t* lists = reinterpret_cast<uint32_t*>(base_ptr + o_lists);
+    uint4* entries = reinterpret_cast<uint4*>(base_ptr + o_entries);
+    uint32_t* llist = reinterpret_cast<uint32_t*>(base_ptr + o_llist);
     uint32_t* tstart = reinterpret_cast<uint32_t*>(base_ptr + o_tstart);
     uint32_t* active = reinterpret_cast<uint32_t*>(base_ptr + o_active);
     uint32_t* astart = reinterpret_cast<uint32_t*>(base_ptr + o_astart);
@@ -626,16 +649,14 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
         std::max<std::size_t>(1, std::min<std::size_t>((nseg + 255) / 256, 1024)));
     hipLaunchKernelGGL(classify_kernel, dim3(cgrid), dim3(256), 0, s, off,
                        static_cast<uint32_t>(nseg), static_cast<uint64_t>(num_items), t,
-                       static_cast<uint32_t>(L_max), counts, lists,
-                       static_cast<uint32_t>(nseg));
+                       static_cast<uint32_t>(L_max), counts, entries, llist);
     for (int c = 0; c < NCFG; ++c)
         if (used[c])
-            launch_lds<K, KIND, DESC, VALUES>(ctx, false, c, keys, vals, off,
-                                              lists + std::size_t(c) * nseg,
-                                              counts + c, lds_q + c);
+            launch_lds<K, KIND, DESC, VALUES>(ctx, false, c, keys, vals,
+                                              entries + std::size_t(c) * nseg, counts + c,
+                                              lds_q + c);
     if (L_max > 0)
     {
-        const uint32_t* llist = lists + std::size_t(LARGE) * nseg;
         hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, s, off, llist,
                            counts + LARGE, static_cast<uint32_t>(L_max),
                            static_cast<uint32_t>(T_max), tstart, large_used, total_tiles);
