@@ -417,25 +417,26 @@ __device__ __forceinline__ void digit_bases(uint32_t* s_hist, uint32_t* s_tmp)
 }
 
 //! One stable LDS counting pass of a block over words w[] (row r of wave
-//! `wave` holds position wave * 64 * ITEMS + r * 64 + lane) by the digit
-//! (w >> shift) & 0xFF: wave-ballot ranks, per-wave digit counters, an LDS
-//! reorder. Only positions below m are keys; rows of a wave that hold none
-//! are skipped, and the other positions past m take digit 255, so they end
-//! behind every key (stability). With MOVE set, values v[] follow the
-//! words through s_v in the same reorder.
+//! `wave` holds position wave * 64 * ITEMS + r * 64 + lane) by the nb-bit
+//! digit (w >> shift) & (2^nb - 1): wave-ballot ranks over nb bits,
+//! per-wave digit counters, an LDS reorder. Only positions below m are
+//! keys; rows of a wave that hold none are skipped, and the other positions
+//! past m take the largest digit, so they end behind every key (stability).
+//! With MOVE set, values v[] follow the words through s_v in the same
+//! reorder. The counters must be zero on entry and are zero again on exit
+//! (cleared while the reordered rows are read back).
 template <typename W, int MOVE, int BLOCK, int ITEMS>
 __device__ __forceinline__ void lds_pass(W (&w)[ITEMS], uint32_t (&v)[ITEMS], uint32_t shift,
-                                         uint32_t m, uint32_t* s_hist, W* s_w, uint32_t* s_v,
-                                         uint32_t* s_tmp)
+                                         uint32_t nb, uint32_t m, uint32_t* s_hist, W* s_w,
+                                         uint32_t* s_v, uint32_t* s_tmp)
 {
     constexpr int NW = BLOCK / WAVE;
     const uint32_t tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
     const uint32_t wave_base = wave * (WAVE * ITEMS);
     const uint32_t live =
         m > wave_base ? min(static_cast<uint32_t>(ITEMS), (m - wave_base + WAVE - 1) / WAVE) : 0u;
+    const uint32_t mask = (1u << nb) - 1u;
     uint32_t* my_hist = s_hist + wave * RADIX;
-    for (uint32_t i = tid; i < NW * RADIX; i += BLOCK) s_hist[i] = 0;
-    __syncthreads();
     uint32_t rank[ITEMS];
     uint32_t dg[(ITEMS + 3) / 4]; // digits, four to a register
 #pragma unroll
@@ -443,7 +444,7 @@ __device__ __forceinline__ void lds_pass(W (&w)[ITEMS], uint32_t (&v)[ITEMS], ui
         if (static_cast<uint32_t>(r) < live)
         {
             const bool valid = wave_base + r * WAVE + lane < m;
-            const uint32_t d = valid ? static_cast<uint32_t>(w[r] >> shift) & 0xFFu : RADIX - 1;
+            const uint32_t d = valid ? static_cast<uint32_t>(w[r] >> shift) & mask : mask;
             dg[r / 4] = r % 4 == 0 ? d : dg[r / 4] | (d << (8 * (r % 4)));
             const uint64_t peers = match_digit(d);
             const uint32_t pre = count_below(peers);
@@ -469,7 +470,26 @@ __device__ __forceinline__ void lds_pass(W (&w)[ITEMS], uint32_t (&v)[ITEMS], ui
             w[r] = s_w[wave_base + r * WAVE + lane];
             if constexpr (MOVE == 1) v[r] = s_v[wave_base + r * WAVE + lane];
         }
+    for (uint32_t i = tid; i < NW * RADIX; i += BLOCK) s_hist[i] = 0;
     __syncthreads();
+}
+
+//! Stable LSD passes of lds_pass over bits [lo, lo + span) of the words:
+//! ceil(span / 8) passes of 8 bits (the last one narrower).
+template <typename W, int MOVE, int BLOCK, int ITEMS>
+__device__ __forceinline__ void lds_passes(W (&w)[ITEMS], uint32_t (&v)[ITEMS], uint32_t lo,
+                                           uint32_t span, uint32_t m, uint32_t* s_hist, W* s_w,
+                                           uint32_t* s_v, uint32_t* s_tmp)
+{
+    const uint32_t passes = (span + 7) / 8;
+    uint32_t shift = lo;
+#pragma unroll 1
+    for (uint32_t q = 0; q < passes; ++q)
+    {
+        const uint32_t nb = min(8u, span - 8 * q);
+        lds_pass<W, MOVE, BLOCK, ITEMS>(w, v, shift, nb, m, s_hist, s_w, s_v, s_tmp);
+        shift += nb;
+    }
 }
 
 //! LDS bytes per key of lds_sort_segment's staging buffer: 64-bit keys
@@ -544,7 +564,8 @@ __device__ __forceinline__ void lds_sort_segment(
         lo = min(lo, static_cast<U>(__shfl_xor(lo, o, WAVE)));
         hi = max(hi, static_cast<U>(__shfl_xor(hi, o, WAVE)));
     }
-    __syncthreads(); // s_min / s_max initialised
+    for (uint32_t i = tid; i < (BLOCK / WAVE) * RADIX; i += BLOCK) s_hist[i] = 0;
+    __syncthreads(); // s_min / s_max initialised, digit counters zero
     if (lane == 0 && wave_base < m)
     {
         atomicMin(&s_min, lo);
@@ -559,7 +580,6 @@ __device__ __forceinline__ void lds_sort_segment(
     }
     const uint32_t bits = WIDE ? 64u - __clzll(static_cast<unsigned long long>(range))
                                : 32u - __clz(static_cast<uint32_t>(range));
-    const uint32_t passes = (bits + 7) / 8;
 
     if constexpr (WIDE && VALUES)
     {
@@ -572,12 +592,10 @@ __device__ __forceinline__ void lds_sort_segment(
 #pragma unroll
         for (int r = 0; r < ITEMS; ++r)
             k[r] = (static_cast<uint64_t>(k[r] - kmin) << 16) | (wave_base + r * WAVE + lane);
-        const uint32_t low_passes = min(passes, 6u);
-#pragma unroll 1
-        for (uint32_t q = 0; q < low_passes; ++q)
-            lds_pass<uint64_t, 0, BLOCK, ITEMS>(k, unused, 16 + 8 * q, m, s_hist, s_w, nullptr,
-                                                s_tmp);
-        if (passes > 6)
+        const bool wide = bits > 48;
+        lds_passes<uint64_t, 0, BLOCK, ITEMS>(k, unused, 16, min(bits, 48u), m, s_hist, s_w,
+                                              nullptr, s_tmp);
+        if (wide)
         {
             // Positions past m fetch position 0 (in bounds, never used).
             K hi16[ITEMS];
@@ -596,10 +614,8 @@ __device__ __forceinline__ void lds_sort_segment(
                     const uint64_t rel = order_bits<K, KIND, DESC>(hi16[r]) - kmin;
                     k[r] = ((rel >> 48) << 16) | (k[r] & 0xFFFFu);
                 }
-#pragma unroll 1
-            for (uint32_t q = 6; q < passes; ++q)
-                lds_pass<uint64_t, 0, BLOCK, ITEMS>(k, unused, 16 + 8 * (q - 6), m, s_hist, s_w,
-                                                    nullptr, s_tmp);
+            lds_passes<uint64_t, 0, BLOCK, ITEMS>(k, unused, 16, bits - 48, m, s_hist, s_w,
+                                                  nullptr, s_tmp);
         }
         // Each value (and, for a wide range, each key) by its original
         // position; all are read before any is overwritten in place (the
@@ -611,7 +627,7 @@ __device__ __forceinline__ void lds_sort_segment(
                 const uint32_t at = wave_base + r * WAVE + lane < m
                     ? static_cast<uint32_t>(k[r]) & 0xFFFFu : 0u;
                 v[r] = vals[at];
-                if (passes > 6) k[r] = keys[at];
+                if (wide) k[r] = keys[at];
                 else k[r] = from_order_bits<K, KIND, DESC>(static_cast<K>(kmin + (k[r] >> 16)));
             }
         __builtin_amdgcn_s_waitcnt(0);
@@ -634,9 +650,7 @@ __device__ __forceinline__ void lds_sort_segment(
     uint32_t* s_v = reinterpret_cast<uint32_t*>(s_w + TILE);
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r) k[r] -= kmin;
-#pragma unroll 1
-    for (uint32_t q = 0; q < passes; ++q)
-        lds_pass<K, MOVE, BLOCK, ITEMS>(k, v, 8 * q, m, s_hist, s_w, s_v, s_tmp);
+    lds_passes<K, MOVE, BLOCK, ITEMS>(k, v, 0, bits, m, s_hist, s_w, s_v, s_tmp);
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r)
     {
