@@ -5,7 +5,7 @@ Main line: stable key-value radix sort (clc_sort) of 2^28 uint32 keys
 uniform over 32 bits with uint32 values 0..n-1, ascending.
 
     python bench.py [--steps K] [--warmup W] [--log2n 28] [--key u32|u64]
-                    [--legs u64,learned,segmented | none] [--no-cpu]
+                    [--legs u64,learned,segmented,batched | none] [--no-cpu]
 
 A step is one clc_sort call on inputs already resident in HBM. Before each
 step the pristine keys/values are restored by a device-to-device copy that
@@ -22,7 +22,12 @@ workloads of the same path, each timed the same way:
   segmented  65,536 segments with lengths round(exp(N(ln 512, 1.5)))
              clamped to [1, 2^20], uint32 keys + segment-local indices:
              one captured clc_graph launch against one clc_sort call per
-             segment; outputs compared bit for bit.
+             segment; outputs compared bit for bit;
+  batched    clc_learned_segmented_sort against clc_segmented_sort on 4096
+             float32 segments with log-uniform sizes 2^10..2^22 (about 2^31
+             pairs), each from uniform[0,1), normal(0,1), exponential(1) or
+             a bimodal normal mix with +-0 and duplicates; outputs compared
+             bit for bit.
 """
 from __future__ import annotations
 
@@ -234,8 +239,11 @@ LEARNED_DISTS = {
 def learned_leg(torch, clc_hip, ctx, log2n, steps, warmup):
     n = 1 << log2n
     kb, vb = 8, 4
+    # learned: id histogram (k), two id passes and the LDS finish (2(k+v)
+    # each); plain: histogram (k) and 8 passes (2(k+v) each)
+    lbytes = kb + 6 * (kb + vb)
     out = {"n": n, "key": "u64", "value": "u32 index",
-           "algorithmic_bytes_per_pair": {"learned": kb + 4 * (kb + vb),
+           "algorithmic_bytes_per_pair": {"learned": lbytes,
                                           "plain": kb + 8 * 2 * (kb + vb)}}
     g = torch.Generator(device="cuda")
     for i, (name, gen) in enumerate(LEARNED_DISTS.items()):
@@ -287,7 +295,7 @@ def learned_leg(torch, clc_hip, ctx, log2n, steps, warmup):
             "learned_gkeys": round(n / res["learned"] / 1e6, 3),
             "plain_gkeys": round(n / res["plain"] / 1e6, 3),
             "speedup": round(res["plain"] / res["learned"], 3),
-            "learned_roofline_frac": round((kb + 4 * (kb + vb)) * n / (res["learned"] / 1e3)
+            "learned_roofline_frac": round(lbytes * n / (res["learned"] / 1e3)
                                            / 1e9 / HBM_PEAK_GBS, 4),
             "bit_identical_to_plain": same,
             "used_model": st["used_model"], "num_buckets": st["num_buckets"],
@@ -301,6 +309,95 @@ def learned_leg(torch, clc_hip, ctx, log2n, steps, warmup):
         say(name, out[name])
         del keys0, vals0, kl, vl, kp, vp
         torch.cuda.empty_cache()
+    return out
+
+
+def c5_keys_gpu(torch, off, g):
+    """float32 keys of the batched learned workload: segment i drawn from
+    uniform[0,1), normal(0,1), exponential(1) or a bimodal normal mix
+    (i mod 4), with +0, -0 and repeated values injected; no NaN."""
+    total = int(off[-1])
+    keys = torch.empty(total, dtype=torch.float32, device="cuda")
+    for i in range(len(off) - 1):
+        b, e = int(off[i]), int(off[i + 1])
+        m = e - b
+        kind = i % 4
+        if kind == 0:
+            x = torch.rand(m, generator=g, device="cuda")
+        elif kind == 1:
+            x = torch.randn(m, generator=g, device="cuda")
+        elif kind == 2:
+            x = torch.empty(m, device="cuda").exponential_(1.0, generator=g)
+        else:
+            pick = torch.rand(m, generator=g, device="cuda") < 0.5
+            lo = torch.randn(m, generator=g, device="cuda") * 0.5 - 3.0
+            hi = torch.randn(m, generator=g, device="cuda") + 4.0
+            x = torch.where(pick, lo, hi)
+        x[::37] = 0.0
+        x[5::41] = -0.0
+        x[7::29] = x[3]
+        keys[b:e] = x
+    return keys
+
+
+def learned_batched_leg(torch, clc_hip, ctx, steps, warmup):
+    """4096 float32 segments, log-uniform sizes 2^10..2^22 (~2^31 pairs in
+    total), u32 values: the batched learned sort against the plain
+    segmented sort on the same GPU, bit-identical outputs."""
+    import numpy as np
+    nseg = 4096
+    rng = np.random.default_rng(4096)
+    sizes = np.rint(2.0 ** rng.uniform(10.0, 22.0, nseg)).astype(np.int64)
+    off = np.zeros(nseg + 1, dtype=np.int64)
+    np.cumsum(sizes, out=off[1:])
+    total = int(off[-1])
+    say("learned batched leg", nseg, "segments,", total, "pairs")
+    g = torch.Generator(device="cuda")
+    g.manual_seed(4096)
+    keys0 = c5_keys_gpu(torch, off, g)
+    vals0 = torch.arange(total, dtype=torch.int32, device="cuda")
+    off_d = torch.from_numpy(off).cuda()
+    kl, vl = torch.empty_like(keys0), torch.empty_like(vals0)
+    kp, vp = torch.empty_like(keys0), torch.empty_like(vals0)
+    res = {}
+    for label, k, v, call in (
+            ("learned", kl, vl, lambda: ctx.learned_segmented_sort(
+                kl, vl, off_d, key_type=clc_hip.F32, stats=False)),
+            ("plain", kp, vp, lambda: ctx.segmented_sort(kp, vp, off_d, key_type=clc_hip.F32))):
+        times = []
+        for it in range(warmup + steps):
+            k.copy_(keys0)
+            v.copy_(vals0)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            call()
+            e1.record()
+            torch.cuda.synchronize()
+            if it >= warmup:
+                times.append(e0.elapsed_time(e1))
+        res[label] = statistics.fmean(times)
+        say("learned batched", label, round(res[label], 3), "ms")
+    same = bool(torch.equal(kl, kp)) and bool(torch.equal(vl, vp))
+    kl.copy_(keys0)
+    vl.copy_(vals0)
+    st = ctx.learned_segmented_sort(kl, vl, off_d, key_type=clc_hip.F32, stats=True)
+    torch.cuda.synchronize()
+    same = same and bool(torch.equal(kl, kp)) and bool(torch.equal(vl, vp))
+    out = {
+        "segments": nseg, "pairs": total, "key": "f32", "value": "u32 index",
+        "min_len": int(sizes.min()), "max_len": int(sizes.max()),
+        "learned_ms": round(res["learned"], 3), "plain_segmented_ms": round(res["plain"], 3),
+        "learned_gkeys": round(total / res["learned"] / 1e6, 3),
+        "plain_gkeys": round(total / res["plain"] / 1e6, 3),
+        "speedup": round(res["plain"] / res["learned"], 3),
+        "bit_identical_to_plain": same,
+        "buckets": st["buckets"], "sample_size": st["sample_size"],
+        "fit_rejected": st["fit_rejected"], "max_bucket": st["max_bucket"],
+        "fallback_fraction": round(st["fallback_fraction"], 5),
+    }
+    del keys0, vals0, kl, vl, kp, vp, off_d
+    torch.cuda.empty_cache()
     return out
 
 
@@ -401,7 +498,7 @@ def main() -> int:
                     help="log2 of the key count (default 28 for u32, 27 for u64)")
     ap.add_argument("--key", choices=["u32", "u64"], default="u32",
                     help="u32: full 32-bit keys; u64: keys uniform over [0, 2^48)")
-    ap.add_argument("--legs", default="u64,learned,segmented",
+    ap.add_argument("--legs", default="u64,learned,segmented,batched",
                     help="extra workloads to report: comma list of u64, learned, "
                          "segmented, or 'none'")
     ap.add_argument("--keys-only", action="store_true",
@@ -417,7 +514,7 @@ def main() -> int:
         return 2
     legs = [] if args.legs == "none" else [x for x in args.legs.split(",") if x]
     for x in legs:
-        if x not in ("u64", "learned", "segmented"):
+        if x not in ("u64", "learned", "segmented", "batched"):
             ap.error(f"unknown leg {x}")
 
     import torch
@@ -490,6 +587,10 @@ def main() -> int:
         sg = segmented_leg(torch, clc_hip, ctx, max(3, args.steps // 2), 2)
         ok = ok and sg["bit_identical"]
         line["legs"]["segmented"] = sg
+    if "batched" in legs:
+        lb = learned_batched_leg(torch, clc_hip, ctx, 3, 1)
+        ok = ok and lb["bit_identical_to_plain"]
+        line["legs"]["learned_batched"] = lb
     line["extra"]["output_check"] = ok
     if not args.no_cpu:
         say("cpu baseline")

@@ -284,6 +284,37 @@ clc_status clc_learned_sort(clc_context ctx, void* keys, clc_type key_type,
                             void* values, size_t n, int descending,
                             clc_learned_stats* stats);
 
+/* Outcome of one clc_learned_segmented_sort call. */
+typedef struct clc_learned_batched_stats
+{
+    /* buckets of the segments sorted through a model, plus every other
+     * segment (sorted whole) */
+    size_t buckets;
+    /* sampled keys over the segments that were modelled */
+    size_t sample_size;
+    /* segments whose model failed the fit check (sorted whole instead) */
+    size_t fit_rejected;
+    /* keys in the largest bucket or whole segment */
+    size_t max_bucket;
+    /* fraction of keys in buckets or segments too large for LDS, finished by
+     * the segmented sort's onesweep path */
+    double fallback_fraction;
+} clc_learned_batched_stats;
+
+/* Same result as clc_segmented_sort (each segment [offsets[i],
+ * offsets[i+1]) sorted stably, values permuted alongside) for 32-bit keys
+ * (U32, I32, F32), through one learned model per segment: segments larger
+ * than the LDS segment sort holds are sampled (1%), their samples sorted by
+ * the segmented sort, one RMI per segment fitted in one launch, keys
+ * grouped into ~3072-key buckets by one or two stable passes (inference
+ * fused in), and the buckets finished by the segmented sort; segments that
+ * fit LDS, or whose sample fits badly, are sorted whole. No host round trip
+ * (one synchronisation to fill stats when stats != NULL). */
+clc_status clc_learned_segmented_sort(clc_context ctx, void* keys, clc_type key_type,
+                                      void* values, size_t num_items,
+                                      const uint64_t* offsets, size_t num_segments,
+                                      int descending, clc_learned_batched_stats* stats);
+
 /* Whether clc_scan / clc_reduce support op/identity for type:
  * CLC_SUCCESS, CLC_ERROR_UNSUPPORTED (expression not recognised) or
  * CLC_ERROR_INVALID_ARGUMENT. Needs no device. */

@@ -63,6 +63,12 @@ class LearnedStats(ctypes.Structure):
                 ("fallback_fraction", ctypes.c_double), ("root_log", ctypes.c_int)]
 
 
+class LearnedBatchedStats(ctypes.Structure):
+    _fields_ = [("buckets", ctypes.c_size_t), ("sample_size", ctypes.c_size_t),
+                ("fit_rejected", ctypes.c_size_t), ("max_bucket", ctypes.c_size_t),
+                ("fallback_fraction", ctypes.c_double)]
+
+
 # Every symbol include/compute_c.h declares, with (restype, argtypes).
 _vp, _sz, _i, _cp = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_char_p
 SIGNATURES = {
@@ -96,6 +102,8 @@ SIGNATURES = {
     "clc_segmented_sort_graph_create": (_i, [_vp, _vp, _i, _vp, _sz, _vp, _sz, _i,
                                              ctypes.POINTER(_vp)]),
     "clc_learned_sort": (_i, [_vp, _vp, _i, _vp, _sz, _i, ctypes.POINTER(LearnedStats)]),
+    "clc_learned_segmented_sort": (_i, [_vp, _vp, _i, _vp, _sz, _vp, _sz, _i,
+                                        ctypes.POINTER(LearnedBatchedStats)]),
     "clc_graph_launch": (_i, [_vp]),
     "clc_graph_release": (None, [_vp]),
 }
@@ -241,6 +249,22 @@ class Context:
         if st is None:
             return None
         return {name: getattr(st, name) for name, _ in LearnedStats._fields_}
+
+    def learned_segmented_sort(self, keys, values, offsets, descending=False, key_type=None,
+                               num_items=None, num_segments=None, stats=True):
+        """clc_learned_segmented_sort; returns its statistics as a dict (or None)."""
+        kt = dtype_code(keys.dtype) if key_type is None else key_type
+        n = keys.numel() if num_items is None else num_items
+        ns = offsets.numel() - 1 if num_segments is None else num_segments
+        st = LearnedBatchedStats() if stats else None
+        check(lib().clc_learned_segmented_sort(
+            self.handle, ctypes.c_void_p(keys.data_ptr()), kt,
+            None if values is None else ctypes.c_void_p(values.data_ptr()), n,
+            ctypes.c_void_p(offsets.data_ptr()), ns, 1 if descending else 0,
+            ctypes.byref(st) if st is not None else None))
+        if st is None:
+            return None
+        return {name: getattr(st, name) for name, _ in LearnedBatchedStats._fields_}
 
     def sort_tune(self, key_type, with_values=True) -> int:
         shape = ctypes.c_int(-1)

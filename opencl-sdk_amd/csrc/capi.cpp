@@ -448,6 +448,28 @@ clc_status clc_learned_sort(clc_context ctx, void* keys, clc_type key_type,
     });
 }
 
+clc_status clc_learned_segmented_sort(clc_context ctx, void* keys, clc_type key_type,
+                                      void* values, size_t num_items,
+                                      const uint64_t* offsets, size_t num_segments,
+                                      int descending, clc_learned_batched_stats* stats)
+{
+    return guarded([&] {
+        require(ctx != nullptr, "clc: null context");
+        clc::learned::batched_stats st;
+        clc::detail::learned_segmented_sort(get(ctx), keys, key_type, values, num_items,
+                                            offsets, num_segments, descending != 0,
+                                            stats != nullptr ? &st : nullptr);
+        if (stats != nullptr)
+        {
+            stats->buckets = st.buckets;
+            stats->sample_size = st.sample_size;
+            stats->fit_rejected = st.fit_rejected;
+            stats->max_bucket = st.max_bucket;
+            stats->fallback_fraction = st.fallback_fraction;
+        }
+    });
+}
+
 clc_status clc_operator_supported(clc_type type, const char* op,
                                   const char* identity)
 {

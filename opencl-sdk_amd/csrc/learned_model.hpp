@@ -1,0 +1,101 @@
+// The learned CDF model shared by the learned-bucket sorts (single array
+// and batched segments): a two-level RMI, one root linear model over a
+// feature of the key choosing one of 256 leaves, one linear CDF model per
+// leaf, and the monotone key -> bucket map it defines.
+#pragma once
+
+#include "device_common.hpp"
+#include "radix_tile.hpp"
+
+#include <cstdint>
+
+namespace clc {
+namespace learned {
+
+constexpr uint32_t LEAVES = 256;
+//! Every SAMPLE_STRIDE-th key (from SAMPLE_STRIDE / 2) forms the sample.
+constexpr uint32_t SAMPLE_STRIDE = 100;
+
+struct rmi_leaf
+{
+    double a, b; // CDF ~ a * x + b
+    uint32_t lo, hi; // bucket range of the leaf
+};
+
+struct rmi_model
+{
+    double a0, b0; // leaf ~ a0 * f + b0, f = root_feature(x)
+    double nb; // number of buckets
+    uint32_t root_log; // root feature: 0 the key, 1 its log image
+    uint32_t pad;
+    rmi_leaf leaf[LEAVES];
+};
+
+//! Log image of an order image: the bits of double(x) read as an integer,
+//! a monotone non-decreasing, piecewise linear approximation of
+//! 2^52 * (log2(x) + 1023). A root linear in it spreads keys that span
+//! many octaves (lognormal, Zipf) over the leaves by magnitude.
+__device__ __forceinline__ uint64_t log_image(uint64_t x)
+{
+    return static_cast<uint64_t>(__double_as_longlong(static_cast<double>(x)));
+}
+
+__device__ __forceinline__ double root_feature(const rmi_model& m, uint64_t bits)
+{
+    return static_cast<double>(m.root_log ? log_image(bits) : bits);
+}
+
+__device__ __forceinline__ uint32_t leaf_of(const rmi_model& m, double f)
+{
+    const double r = fma(m.a0, f, m.b0);
+    return r <= 0.0 ? 0u : r >= double(LEAVES - 1) ? LEAVES - 1 : static_cast<uint32_t>(r);
+}
+
+//! Monotone non-decreasing bucket of an order image x: the root feature,
+//! its conversion to double and every model step are monotone, leaf slopes
+//! are >= 0 and leaf bucket ranges are ordered.
+__device__ __forceinline__ uint32_t bucket_of(const rmi_model& m, uint64_t bits)
+{
+    const double x = static_cast<double>(bits);
+    const rmi_leaf& L = m.leaf[leaf_of(m, root_feature(m, bits))];
+    const double p = fma(L.a, x, L.b) * m.nb;
+    const double lo = L.lo, hi = L.hi;
+    return p <= lo ? L.lo : p >= hi ? L.hi : static_cast<uint32_t>(p);
+}
+
+//! Onesweep digit functor: byte shift/8 of the key's bucket under model m.
+template <typename K, int KIND, bool DESC> struct id_digit
+{
+    //! Inference is costly: one evaluation per key and pass, the digit
+    //! staged in s_dig for the write-out.
+    static constexpr bool STAGED = true;
+    const rmi_model* m;
+    uint32_t shift;
+    uint8_t* s_dig;
+    __device__ __forceinline__ uint32_t operator()(K k) const
+    {
+        return (bucket_of(*m, static_cast<uint64_t>(dev::order_bits<K, KIND, DESC>(k))) >> shift)
+            & 0xFFu;
+    }
+};
+
+__device__ __forceinline__ void load_model(rmi_model* s_m, const rmi_model* g_m)
+{
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(g_m);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(s_m);
+    for (uint32_t i = threadIdx.x; i < sizeof(rmi_model) / 4; i += blockDim.x) dst[i] = src[i];
+}
+
+//! Slope (>= 0) and intercept of y on x from moments (y = a x + c).
+__device__ inline void line_from_moments(const double* mo, double& a, double& c)
+{
+    const double n = mo[0];
+    const double mx = mo[1] / n, my = mo[2] / n;
+    const double var = mo[3] / n - mx * mx;
+    const double cov = mo[4] / n - mx * my;
+    a = var > 0 ? fmax(cov / var, 0.0) : 0.0;
+    c = my - a * mx;
+}
+
+} // namespace learned
+} // namespace clc

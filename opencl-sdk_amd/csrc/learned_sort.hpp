@@ -24,6 +24,16 @@ struct stats
     int root_log = 0; //!< 1 when the root is linear in the key's log image
 };
 
+//! Outcome of one batched learned sort (mirrors clc_learned_batched_stats).
+struct batched_stats
+{
+    std::size_t buckets = 0; //!< buckets of fitted segments plus segments sorted whole
+    std::size_t sample_size = 0; //!< sampled keys over all fitted segments
+    std::size_t fit_rejected = 0; //!< segments whose fit check failed (sorted whole)
+    std::size_t max_bucket = 0; //!< keys in the largest bucket or whole segment
+    double fallback_fraction = 0; //!< keys in buckets or segments too large for LDS
+};
+
 } // namespace learned
 
 namespace detail {
@@ -32,6 +42,15 @@ namespace detail {
 //! fills out when given (which costs one extra synchronisation).
 void learned_sort(context& ctx, void* keys, int key_type, void* values, std::size_t n,
                   bool descending, learned::stats* out);
+
+//! Stable sort of every segment [offsets[i], offsets[i+1]) of 32-bit keys
+//! (and values) with one learned model per segment (learned_batched.hip);
+//! same result as segmented_sort. Fills out when given (one extra
+//! synchronisation).
+void learned_segmented_sort(context& ctx, void* keys, int key_type, void* values,
+                            std::size_t num_items, const uint64_t* offsets,
+                            std::size_t num_segments, bool descending,
+                            learned::batched_stats* out);
 
 } // namespace detail
 } // namespace clc
