@@ -60,7 +60,7 @@ constexpr uint32_t BUCKET_TARGET = 3072;
 constexpr uint32_t MIN_LEARNED = 1u << 15;
 constexpr uint32_t MAX_BUCKETS = 65536; // 16-bit bucket ids: two 8-bit passes
 //! Onesweep tile of the id passes (split staging, digits staged in LDS).
-constexpr int BBLOCK = 1024, BITEMS = 14;
+constexpr int BBLOCK = 1024, BITEMS = 18;
 constexpr uint32_t BTILE = BBLOCK * BITEMS;
 
 //! Per segment, filled by prepare_kernel.
@@ -320,36 +320,62 @@ __device__ __forceinline__ uint32_t segment_of_tile(const seg_info* info, uint32
 }
 
 //! Per learned segment (j), counts of the bucket id's low and high byte:
-//! hist[j][0][d], hist[j][1][d]. One block per tile.
+//! hist[j][0][d], hist[j][1][d]. Block b walks tiles [b * HU, b * HU + HU)
+//! in order, counting in LDS and adding to global memory when the segment
+//! changes or at the end (one flush per segment piece, not per tile), with
+//! HL independent loads per thread in flight.
+constexpr uint32_t HU = 8, HL = 4;
 template <int KIND, bool DESC>
-__global__ __launch_bounds__(256) void id_hist_kernel(const uint32_t* __restrict__ keys,
-                                                      const seg_info* __restrict__ info,
-                                                      uint32_t nseg,
-                                                      const rmi_model* __restrict__ models,
-                                                      const uint32_t* totals,
-                                                      uint32_t* __restrict__ hist)
+__global__ __launch_bounds__(1024) void id_hist_kernel(const uint32_t* __restrict__ keys,
+                                                       const seg_info* __restrict__ info,
+                                                       uint32_t nseg,
+                                                       const rmi_model* __restrict__ models,
+                                                       const uint32_t* totals,
+                                                       uint32_t* __restrict__ hist)
 {
     __shared__ rmi_model sm;
     __shared__ uint32_t s_h[2 * RADIX];
-    const uint32_t t = blockIdx.x;
-    if (t >= totals[2]) return;
-    const uint32_t j = segment_of_tile(info, nseg, t);
-    const seg_info si = info[j];
-    if (!si.learned) return; // fit check failed: sorted whole
-    load_model(&sm, models + j);
-    for (uint32_t i = threadIdx.x; i < 2 * RADIX; i += 256) s_h[i] = 0;
-    __syncthreads();
-    const uint32_t lo = (t - si.tile) * BTILE, hi = min(si.m, lo + BTILE);
-    const uint32_t* in = keys + si.begin;
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += 256)
+    const uint32_t T = totals[2];
+    const uint32_t t0 = blockIdx.x * HU, t1 = min(T, t0 + HU);
+    uint32_t cur = 0xFFFFFFFFu; // segment in LDS
+    for (uint32_t t = t0; t < t1; ++t)
     {
-        const uint32_t b = bucket_of(sm, order_bits<uint32_t, KIND, DESC>(in[i]));
-        atomicAdd(&s_h[b & 0xFFu], 1u);
-        atomicAdd(&s_h[RADIX + (b >> 8)], 1u);
+        const uint32_t j = segment_of_tile(info, nseg, t);
+        const seg_info si = info[j];
+        if (!si.learned) continue; // uniform: every thread reads the same info
+        if (j != cur)
+        {
+            __syncthreads(); // previous counts complete
+            if (cur != 0xFFFFFFFFu)
+                for (uint32_t i = threadIdx.x; i < 2 * RADIX; i += 1024)
+                    if (s_h[i] != 0) atomicAdd(&hist[size_t(cur) * 2 * RADIX + i], s_h[i]);
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < 2 * RADIX; i += 1024) s_h[i] = 0;
+            load_model(&sm, models + j);
+            cur = j;
+            __syncthreads();
+        }
+        const uint32_t lo = (t - si.tile) * BTILE, hi = min(si.m, lo + BTILE);
+        const uint32_t* in = keys + si.begin;
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += HL * 1024)
+        {
+            uint32_t k[HL];
+#pragma unroll
+            for (uint32_t u = 0; u < HL; ++u) k[u] = in[min(i + u * 1024, hi - 1)];
+#pragma unroll
+            for (uint32_t u = 0; u < HL; ++u)
+                if (i + u * 1024 < hi)
+                {
+                    const uint32_t b = bucket_of(sm, order_bits<uint32_t, KIND, DESC>(k[u]));
+                    atomicAdd(&s_h[b & 0xFFu], 1u);
+                    atomicAdd(&s_h[RADIX + (b >> 8)], 1u);
+                }
+        }
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < 2 * RADIX; i += 256)
-        if (s_h[i] != 0) atomicAdd(&hist[size_t(j) * 2 * RADIX + i], s_h[i]);
+    if (cur != 0xFFFFFFFFu)
+        for (uint32_t i = threadIdx.x; i < 2 * RADIX; i += 1024)
+            if (s_h[i] != 0) atomicAdd(&hist[size_t(cur) * 2 * RADIX + i], s_h[i]);
 }
 
 //! Block per (segment, place): exclusive scan of the 256 counts.
@@ -588,8 +614,8 @@ void run(context& ctx, uint32_t* keys, uint32_t* vals, std::size_t n, const uint
     const uint32_t tgrid = static_cast<uint32_t>(tiles_max);
     {
         timed_launch timer(ctx, "learned_batched_id_histogram");
-        hipLaunchKernelGGL((id_hist_kernel<KIND, DESC>), dim3(tgrid), dim3(256), 0, s, keys, info,
-                           nseg, models, totals, hist);
+        hipLaunchKernelGGL((id_hist_kernel<KIND, DESC>), dim3(tgrid / HU + 1), dim3(1024), 0, s,
+                           keys, info, nseg, models, totals, hist);
     }
     hipLaunchKernelGGL(id_scan_kernel, dim3(nseg * 2), dim3(256), 0, s, hist, base);
     const std::size_t lds = std::size_t(BBLOCK / WAVE) * 1024 + RADIX * 4
