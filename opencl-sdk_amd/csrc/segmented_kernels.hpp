@@ -117,7 +117,7 @@ __global__ __launch_bounds__(BLOCK) void lds_class_kernel(
 //! large segments in use.
 __global__ __launch_bounds__(1024) void tile_scan_kernel(
     const uint64_t* __restrict__ off, const uint32_t* __restrict__ large_list,
-    const uint32_t* large_count, uint32_t large_cap, uint32_t max_tiles,
+    const uint32_t* large_count, uint32_t large_cap, uint32_t max_tiles, uint32_t tile,
     uint32_t* __restrict__ tile_start, uint32_t* large_used, uint32_t* total_tiles)
 {
     __shared__ uint32_t s_wave[16];
@@ -132,7 +132,7 @@ __global__ __launch_bounds__(1024) void tile_scan_kernel(
         {
             const uint32_t s = large_list[i];
             const uint64_t m = off[s + 1] - off[s];
-            nt = static_cast<uint32_t>((m + LARGE_TILE - 1) / LARGE_TILE);
+            nt = static_cast<uint32_t>((m + tile - 1) / tile);
         }
         const uint32_t inc = wave_inclusive_add(nt);
         if (lane == WAVE - 1) s_wave[wave] = inc;
@@ -180,6 +180,7 @@ __global__ __launch_bounds__(256) void large_hist_kernel(
     uint32_t* __restrict__ hist, uint32_t* queue)
 {
     constexpr int P = sizeof(K);
+    constexpr uint32_t TILE = large_tile(sizeof(K));
     constexpr int C = 32 / P; // interleaved copies per bin, as in the plain histogram
     __shared__ uint32_t s_h[P * RADIX * C];
     __shared__ uint32_t s_item, s_j;
@@ -201,8 +202,8 @@ __global__ __launch_bounds__(256) void large_hist_kernel(
         const uint32_t sidx = large_list[j];
         const uint64_t b = off[sidx];
         const uint64_t m = off[sidx + 1] - b;
-        const uint64_t lo = uint64_t(t - tile_start[j]) * LARGE_TILE;
-        const uint64_t hi = std::min<uint64_t>(lo + LARGE_TILE, m);
+        const uint64_t lo = uint64_t(t - tile_start[j]) * TILE;
+        const uint64_t hi = std::min<uint64_t>(lo + TILE, m);
         // Digits equal to the tile's first key's (a place constant across
         // the segment, common for large segments) are counted in registers
         // and added once: no LDS atomics all on one address.
@@ -282,8 +283,8 @@ __global__ __launch_bounds__(256) void large_digit_scan_kernel(
 __global__ __launch_bounds__(1024) void active_scan_kernel(
     const uint64_t* __restrict__ off, const uint32_t* __restrict__ large_list,
     const uint32_t* large_used, const uint32_t* __restrict__ exec, uint32_t P,
-    uint32_t max_tiles, uint32_t* __restrict__ active, uint32_t* __restrict__ act_start,
-    uint32_t* act_used, uint32_t* act_tiles)
+    uint32_t max_tiles, uint32_t tile, uint32_t* __restrict__ active,
+    uint32_t* __restrict__ act_start, uint32_t* act_used, uint32_t* act_tiles)
 {
     __shared__ uint32_t s_f[16], s_t[16];
     const uint32_t tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
@@ -299,7 +300,7 @@ __global__ __launch_bounds__(1024) void active_scan_kernel(
             if (f)
             {
                 const uint32_t s = large_list[j];
-                nt = static_cast<uint32_t>((off[s + 1] - off[s] + LARGE_TILE - 1) / LARGE_TILE);
+                nt = static_cast<uint32_t>((off[s + 1] - off[s] + tile - 1) / tile);
             }
         }
         const uint32_t inc_f = wave_inclusive_add(f), inc_t = wave_inclusive_add(nt);
@@ -349,6 +350,7 @@ __global__ __launch_bounds__(256) void large_copy_back_kernel(
     const uint32_t* total_tiles, const uint32_t* __restrict__ exec, uint32_t P,
     uint32_t* queue)
 {
+    constexpr uint32_t TILE = large_tile(sizeof(K));
     __shared__ uint32_t s_item, s_j;
     const uint32_t L = *large_used, T = *total_tiles;
     for (;;)
@@ -370,8 +372,8 @@ __global__ __launch_bounds__(256) void large_copy_back_kernel(
         const uint32_t sidx = large_list[jf];
         const uint64_t b = off[sidx];
         const uint64_t m = off[sidx + 1] - b;
-        const uint64_t lo = uint64_t(t - tile_start[j]) * LARGE_TILE;
-        const uint64_t hi = std::min<uint64_t>(lo + LARGE_TILE, m);
+        const uint64_t lo = uint64_t(t - tile_start[j]) * TILE;
+        const uint64_t hi = std::min<uint64_t>(lo + TILE, m);
         for (uint64_t i = lo + threadIdx.x; i < hi; i += 256)
         {
             keys[b + i] = keys_alt[b + i];
@@ -382,10 +384,13 @@ __global__ __launch_bounds__(256) void large_copy_back_kernel(
 
 //! Persistent onesweep over the tiles of every active large segment (tile
 //! layout and index j from active_scan_kernel; jf = active[j] indexes the
-//! large list), one digit place; look-back chains run within a segment. Per segment, places whose
-//! digit is constant are skipped (exec[j][p] == 0) and each executed place
-//! reads what the previous executed place wrote (a = keys, b = scratch).
-//! A place that no segment runs only clears the next place's status words.
+//! large list), one digit place; look-back chains run within a segment.
+//! Per segment, places whose digit is constant are skipped (exec[j][p] ==
+//! 0) and each executed place reads what the previous executed place wrote
+//! (a = keys, b = scratch). A place that no segment runs only clears the
+//! next place's status words. Persistent rather than one workgroup per
+//! tile: the tile count is only known on the device, and a grid sized for
+//! the worst case would spend more on empty workgroups than on the tiles.
 template <typename K, int KIND, bool DESC, bool VALUES>
 __global__ __launch_bounds__(LARGE_BLOCK) void large_onesweep_kernel(
     K* keys_a, K* keys_b, uint32_t* vals_a, uint32_t* vals_b,
@@ -398,10 +403,13 @@ __global__ __launch_bounds__(LARGE_BLOCK) void large_onesweep_kernel(
 {
     constexpr int P = sizeof(K);
     constexpr int W = LARGE_BLOCK / WAVE;
-    __shared__ uint32_t s_hist[W * RADIX];
-    __shared__ K s_keys[LARGE_TILE];
-    __shared__ uint32_t s_vals[VALUES ? LARGE_TILE : 1];
-    __shared__ uint32_t s_gbase[RADIX];
+    constexpr int ITEMS = large_items(sizeof(K));
+    constexpr uint32_t TILE = large_tile(sizeof(K));
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint32_t* s_hist = reinterpret_cast<uint32_t*>(smem);
+    uint32_t* s_gbase = s_hist + W * RADIX;
+    K* s_keys = reinterpret_cast<K*>(s_gbase + RADIX);
+    uint32_t* s_vals = reinterpret_cast<uint32_t*>(s_keys); // split staging
     __shared__ uint32_t s_tmp[4];
     __shared__ uint32_t s_item, s_j;
     const uint32_t tid = threadIdx.x;
@@ -414,6 +422,7 @@ __global__ __launch_bounds__(LARGE_BLOCK) void large_onesweep_kernel(
                 status_next[i] = 0;
         return;
     }
+#pragma unroll 1
     for (;;)
     {
         if (tid == 0)
@@ -448,12 +457,12 @@ __global__ __launch_bounds__(LARGE_BLOCK) void large_onesweep_kernel(
         const uint32_t* vi = VALUES ? vals_in + b : nullptr;
         uint32_t* vo = VALUES ? vals_out + b : nullptr;
         const key_digit<K, KIND, DESC> dig{ 8u * pass };
-        if (local * LARGE_TILE + LARGE_TILE <= m)
-            onesweep_tile<K, uint32_t, VALUES, LARGE_BLOCK, LARGE_ITEMS, true, false>(
+        if (local * TILE + TILE <= m)
+            onesweep_tile<K, uint32_t, VALUES, LARGE_BLOCK, ITEMS, true, true>(
                 keys_in + b, keys_out + b, vi, vo, m, dig, dbase, st, local, s_hist, s_keys,
                 s_vals, s_gbase, s_tmp);
         else
-            onesweep_tile<K, uint32_t, VALUES, LARGE_BLOCK, LARGE_ITEMS, false, false>(
+            onesweep_tile<K, uint32_t, VALUES, LARGE_BLOCK, ITEMS, false, true>(
                 keys_in + b, keys_out + b, vi, vo, m, dig, dbase, st, local, s_hist, s_keys,
                 s_vals, s_gbase, s_tmp);
         __syncthreads();
@@ -560,6 +569,22 @@ template <typename F> uint32_t cached_grid(const context& ctx, F kernel, int blo
     return resident_grid(ctx, kernel, block, 0);
 }
 
+//! Opts the large path's onesweep kernel in to more than 64 KiB of LDS,
+//! once per kernel, outside any stream capture.
+template <typename F> void large_lds_opt_in(F kernel, std::size_t lds)
+{
+    static std::mutex m;
+    static std::map<const void*, bool> opted; // per kernel (one type, many kernels)
+    {
+        std::lock_guard<std::mutex> lock(m);
+        const void* key = reinterpret_cast<const void*>(kernel);
+        if (!opted[key] && lds > 65536)
+            CLC_HIP(hipFuncSetAttribute(key, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        static_cast<int>(lds)));
+        opted[key] = true;
+    }
+}
+
 //! Enqueues the whole segmented sort on ctx's stream. With prepare set it
 //! only computes the scratch size and the persistent grids (so a later
 //! stream capture needs no runtime queries) and returns the size.
@@ -582,7 +607,8 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
         first_large < NBUCKET ? (std::size_t(1) << (first_large - 1)) + 1 : 0;
     const std::size_t L_max =
         first_large < NBUCKET ? std::min(nseg, num_items / m_min_large) : 0;
-    const std::size_t T_max = L_max ? (num_items + LARGE_TILE - 1) / LARGE_TILE + L_max : 0;
+    constexpr uint32_t TILE = large_tile(sizeof(K));
+    const std::size_t T_max = L_max ? (num_items + TILE - 1) / TILE + L_max : 0;
 
     layout Lt;
     const std::size_t o_ctr = Lt.take(256);
@@ -604,7 +630,9 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
     auto ok = large_onesweep_kernel<K, KIND, DESC, VALUES>;
     auto ck = large_copy_back_kernel<K, VALUES>;
     const uint32_t hgrid = L_max ? cached_grid(ctx, hk, 256) : 0;
-    const uint32_t ogrid = L_max ? cached_grid(ctx, ok, LARGE_BLOCK) : 0;
+    const std::size_t olds = large_lds(sizeof(K));
+    if (L_max) large_lds_opt_in(ok, olds);
+    const uint32_t ogrid = L_max ? resident_grid(ctx, ok, LARGE_BLOCK, olds) : 0;
     const uint32_t cpgrid = L_max ? cached_grid(ctx, ck, 256) : 0;
     if (prepare)
     {
@@ -659,22 +687,22 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
     {
         hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, s, off, llist,
                            counts + LARGE, static_cast<uint32_t>(L_max),
-                           static_cast<uint32_t>(T_max), tstart, large_used, total_tiles);
+                           static_cast<uint32_t>(T_max), TILE, tstart, large_used, total_tiles);
         hipLaunchKernelGGL(hk, dim3(hgrid), dim3(256), 0, s, keys, off, llist, tstart,
                            large_used, total_tiles, hist, hist_q);
         const uint32_t sgrid = static_cast<uint32_t>(std::min<std::size_t>(L_max * P, 4096));
         hipLaunchKernelGGL(large_digit_scan_kernel, dim3(sgrid), dim3(256), 0, s, hist,
                            dbase, large_used, uint32_t(P), off, llist, exec, any_exec);
         hipLaunchKernelGGL(active_scan_kernel, dim3(1), dim3(1024), 0, s, off, llist,
-                           large_used, exec, uint32_t(P), static_cast<uint32_t>(T_max), active,
-                           astart, act_used, act_tiles);
+                           large_used, exec, uint32_t(P), static_cast<uint32_t>(T_max), TILE,
+                           active, astart, act_used, act_tiles);
         K* kb[2] = { keys, reinterpret_cast<K*>(base_ptr + o_kalt) };
         uint32_t* vb[2] = { vals, VALUES ? reinterpret_cast<uint32_t*>(base_ptr + o_valt)
                                          : nullptr };
         for (int p = 0; p < P; ++p)
         {
             timed_launch timer(ctx, "segmented_onesweep");
-            hipLaunchKernelGGL(ok, dim3(ogrid), dim3(LARGE_BLOCK), 0, s, kb[0], kb[1],
+            hipLaunchKernelGGL(ok, dim3(ogrid), dim3(LARGE_BLOCK), olds, s, kb[0], kb[1],
                                vb[0], vb[1], off, llist, active, astart, act_used,
                                act_tiles, uint32_t(p), dbase, exec, any_exec, st[p % 2],
                                p + 1 < P ? st[(p + 1) % 2] : nullptr, pass_q + p);

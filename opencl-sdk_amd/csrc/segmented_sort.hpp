@@ -44,8 +44,22 @@ constexpr int LARGE = NCFG;
 //! Segment-size buckets: bucket b holds 2^(b-1) < m <= 2^b (m >= 2).
 constexpr int NBUCKET = 32;
 //! Onesweep tile of the large path.
-constexpr int LARGE_BLOCK = 256, LARGE_ITEMS = 16;
-constexpr uint32_t LARGE_TILE = LARGE_BLOCK * LARGE_ITEMS;
+constexpr int LARGE_BLOCK = 1024;
+//! Keys per thread of a large-path onesweep tile: as many as the persistent
+//! kernel keeps in 128 VGPRs without spilling.
+constexpr int large_items(std::size_t kbytes) { return kbytes == 8 ? 8 : 12; }
+constexpr uint32_t large_tile(std::size_t kbytes)
+{
+    return static_cast<uint32_t>(LARGE_BLOCK * large_items(kbytes));
+}
+
+//! Dynamic LDS bytes of the large path's onesweep tile: per-wave digit
+//! counters, digit bases, and one staging buffer that the keys and then the
+//! values pass through (split staging).
+inline std::size_t large_lds(std::size_t kbytes)
+{
+    return std::size_t(LARGE_BLOCK / 64) * 1024 + 256 * 4 + large_tile(kbytes) * kbytes + 64;
+}
 
 inline std::size_t cfg_capacity(int c)
 {
