@@ -23,11 +23,11 @@ workloads of the same path, each timed the same way:
              clamped to [1, 2^20], uint32 keys + segment-local indices:
              one captured clc_graph launch against one clc_sort call per
              segment; outputs compared bit for bit;
-  batched    clc_learned_segmented_sort against clc_segmented_sort on 4096
-             float32 segments with log-uniform sizes 2^10..2^22 (about 2^31
-             pairs), each from uniform[0,1), normal(0,1), exponential(1) or
-             a bimodal normal mix with +-0 and duplicates; outputs compared
-             bit for bit.
+  batched    clc_learned_segmented_sort against clc_segmented_sort and
+             against one clc_sort call per segment on 4096 float32 segments
+             with log-uniform sizes 2^10..2^22 (about 2^31 pairs), each from
+             uniform[0,1), normal(0,1), exponential(1) or a bimodal normal
+             mix with +-0 and duplicates; outputs compared bit for bit.
 """
 from __future__ import annotations
 
@@ -379,6 +379,20 @@ def learned_batched_leg(torch, clc_hip, ctx, steps, warmup):
         res[label] = statistics.fmean(times)
         say("learned batched", label, round(res[label], 3), "ms")
     same = bool(torch.equal(kl, kp)) and bool(torch.equal(vl, vp))
+    # The reference's way: one plain sort call per segment (timed once).
+    kp.copy_(keys0)
+    vp.copy_(vals0)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for i in range(nseg):
+        b, e = int(off[i]), int(off[i + 1])
+        ctx.sort(kp[b:e], vp[b:e], key_type=clc_hip.F32)
+    e1.record()
+    torch.cuda.synchronize()
+    res["per_segment"] = e0.elapsed_time(e1)
+    same = same and bool(torch.equal(kl, kp)) and bool(torch.equal(vl, vp))
     kl.copy_(keys0)
     vl.copy_(vals0)
     st = ctx.learned_segmented_sort(kl, vl, off_d, key_type=clc_hip.F32, stats=True)
@@ -391,6 +405,8 @@ def learned_batched_leg(torch, clc_hip, ctx, steps, warmup):
         "learned_gkeys": round(total / res["learned"] / 1e6, 3),
         "plain_gkeys": round(total / res["plain"] / 1e6, 3),
         "speedup": round(res["plain"] / res["learned"], 3),
+        "per_segment_plain_ms": round(res["per_segment"], 3),
+        "speedup_vs_per_segment": round(res["per_segment"] / res["learned"], 3),
         "bit_identical_to_plain": same,
         "buckets": st["buckets"], "sample_size": st["sample_size"],
         "fit_rejected": st["fit_rejected"], "max_bucket": st["max_bucket"],
