@@ -50,8 +50,8 @@ using namespace dev;
 using namespace radix;
 
 // Onesweep tile of the id passes: 1024 threads x 14 keys with split staging
-// and the digits staged as bytes (the largest tile whose LDS fits 64-bit
-// keys with the model).
+// and the digits staged as bytes (the largest tile whose LDS, with the
+// model and the kernel's static LDS, fits 160 KiB for 64-bit keys).
 constexpr int LBLOCK = 1024, LITEMS = 14;
 constexpr uint32_t LTILE = LBLOCK * LITEMS;
 
@@ -61,9 +61,20 @@ template <typename K, int KIND, bool DESC>
 __global__ __launch_bounds__(256) void sample_kernel(const K* __restrict__ keys, uint32_t m,
                                                      uint64_t* __restrict__ out)
 {
-    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < m; i += gridDim.x * 256)
-        out[i] = static_cast<uint64_t>(
-            order_bits<K, KIND, DESC>(keys[size_t(i) * SAMPLE_STRIDE + SAMPLE_STRIDE / 2]));
+    // Four strided reads per thread in flight (each its own cache line).
+    constexpr uint32_t U = 4;
+    const uint32_t stride = gridDim.x * 256;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < m; i += U * stride)
+    {
+        K k[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+            k[u] = keys[size_t(min(i + u * stride, m - 1)) * SAMPLE_STRIDE + SAMPLE_STRIDE / 2];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+            if (i + u * stride < m)
+                out[i + u * stride] = static_cast<uint64_t>(order_bits<K, KIND, DESC>(k[u]));
+    }
 }
 
 //! Least-squares fits use the moments of (x, y) = (key - xref, index -
@@ -361,7 +372,7 @@ __global__ __launch_bounds__(256) void id_histogram_kernel(const K* __restrict__
     {
         K k[U];
 #pragma unroll
-        for (uint32_t u = 0; u < U; ++u) k[u] = i + u * stride < n ? keys[i + u * stride] : K(0);
+        for (uint32_t u = 0; u < U; ++u) k[u] = keys[min(i + u * stride, n - 1)]; // unconditional
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u)
         {

@@ -95,12 +95,18 @@ __global__ __launch_bounds__(BLOCK) void digit_histogram_kernel(
         constexpr uint32_t PER = 16 / sizeof(K);
         const uint32_t nvec = n / PER;
         const uint4* v = reinterpret_cast<const uint4*>(keys);
-        for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < nvec; i += stride)
+        // Two independent 16-byte loads per thread in flight.
+        for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < nvec; i += 2 * stride)
         {
-            const uint4 q = v[i];
-            const K* ks = reinterpret_cast<const K*>(&q);
+            const uint4 q0 = v[i];
+            const uint4 q1 = v[min(i + stride, nvec - 1)];
+            const K* k0 = reinterpret_cast<const K*>(&q0);
+            const K* k1 = reinterpret_cast<const K*>(&q1);
 #pragma unroll
-            for (uint32_t j = 0; j < PER; ++j) count(ks[j]);
+            for (uint32_t j = 0; j < PER; ++j) count(k0[j]);
+            if (i + stride < nvec)
+#pragma unroll
+                for (uint32_t j = 0; j < PER; ++j) count(k1[j]);
         }
         start = nvec * PER;
     }
