@@ -64,12 +64,13 @@ template <typename K, int KIND, bool DESC> struct key_digit
 
 //! Decoupled look-back of one tile over a window of predecessors. Wave w
 //! owns digits [w * DPW, (w + 1) * DPW); lane (dd, j) = (lane / J, lane % J)
-//! reads digit w * DPW + dd of the tile `consumed + j + 1` places back, so
-//! one round of parallel loads covers J = BLOCK / 256 predecessors of every
-//! digit (a single-thread walk would take J round trips). A digit sums the
-//! words before the first unpublished one and stops at its first PREFIX;
-//! it then publishes this tile's inclusive PREFIX (total from s_total) and
-//! adds its exclusive prefix to s_gbase. Every wave of the block calls it.
+//! reads digit w * DPW + dd of the tiles `consumed + q * J + j + 1` places
+//! back for q < R, so one round of R parallel loads per lane covers the
+//! WIN = J * R = 16 nearest unsummed predecessors of every digit (a
+//! single-thread walk would take WIN round trips). A digit sums the words
+//! before the first unpublished one and stops at its first PREFIX; it then
+//! publishes this tile's inclusive PREFIX (total from s_total) and adds its
+//! exclusive prefix to s_gbase. Every wave of the block calls it.
 template <typename S, int BLOCK>
 __device__ __forceinline__ void lookback_window(S* status, uint32_t tile,
                                                 const uint32_t* s_total, uint32_t* s_gbase)
@@ -77,6 +78,8 @@ __device__ __forceinline__ void lookback_window(S* status, uint32_t tile,
     using SB = status_bits<S>;
     constexpr uint32_t J = BLOCK / RADIX;
     constexpr uint32_t DPW = WAVE / J;
+    constexpr uint32_t R = 16 / J;
+    constexpr uint32_t WIN = J * R;
     static_assert(J * RADIX == BLOCK && (J == 1 || J == 2 || J == 4), "block of 256/512/1024");
     const uint32_t lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
     const uint32_t dd = lane / J, j = lane % J;
@@ -86,18 +89,32 @@ __device__ __forceinline__ void lookback_window(S* status, uint32_t tile,
     bool done = tile == 0;
     while (__ballot(!done) != 0)
     {
-        S s = SB::PREFIX; // before tile 0: an empty prefix
-        const uint32_t back = consumed + j + 1;
-        if (!done && back <= tile)
-            s = load_agent(status + static_cast<size_t>(tile - back) * RADIX + d);
-        const uint64_t pm = __ballot(!done && (s & SB::PREFIX) != 0);
-        const uint64_t zm = __ballot(!done && (s & SB::FLAGS) == 0);
-        const uint32_t gp = static_cast<uint32_t>(pm >> (dd * J)) & ((1u << J) - 1);
-        const uint32_t gz = static_cast<uint32_t>(zm >> (dd * J)) & ((1u << J) - 1);
-        const uint32_t jp = gp ? __builtin_ctz(gp) : J; // first PREFIX in the window
-        const uint32_t jz = gz ? __builtin_ctz(gz) : J; // first unpublished word
-        const uint32_t take = jz < jp ? jz : (jp < J ? jp + 1 : J);
-        S part = j < take ? (s & SB::VALUE) : S(0);
+        S w[R];
+#pragma unroll
+        for (uint32_t q = 0; q < R; ++q)
+        {
+            w[q] = SB::PREFIX; // before tile 0: an empty prefix
+            const uint32_t back = consumed + q * J + j + 1;
+            if (!done && back <= tile)
+                w[q] = load_agent(status + static_cast<size_t>(tile - back) * RADIX + d);
+        }
+        // Window position p = q * J + j: bit p of the digit's masks.
+        uint32_t gp = 0, gz = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < R; ++q)
+        {
+            const uint64_t pm = __ballot(!done && (w[q] & SB::PREFIX) != 0);
+            const uint64_t zm = __ballot(!done && (w[q] & SB::FLAGS) == 0);
+            gp |= (static_cast<uint32_t>(pm >> (dd * J)) & ((1u << J) - 1)) << (q * J);
+            gz |= (static_cast<uint32_t>(zm >> (dd * J)) & ((1u << J) - 1)) << (q * J);
+        }
+        const uint32_t jp = gp ? __builtin_ctz(gp) : WIN; // first PREFIX in the window
+        const uint32_t jz = gz ? __builtin_ctz(gz) : WIN; // first unpublished word
+        const uint32_t take = jz < jp ? jz : (jp < WIN ? jp + 1 : WIN);
+        S part = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < R; ++q)
+            if (q * J + j < take) part += w[q] & SB::VALUE;
         if constexpr (J >= 2) part += __shfl_xor(part, 1);
         if constexpr (J >= 4) part += __shfl_xor(part, 2);
         if (!done)
@@ -107,7 +124,7 @@ __device__ __forceinline__ void lookback_window(S* status, uint32_t tile,
             done = jp < jz;
         }
         // A digit stopped at an unpublished word: back off briefly.
-        if (__ballot(!done && jz < J) != 0) __builtin_amdgcn_s_sleep(1);
+        if (__ballot(!done && jz < WIN) != 0) __builtin_amdgcn_s_sleep(1);
     }
     if (j == 0)
     {

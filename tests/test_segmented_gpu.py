@@ -44,7 +44,8 @@ def check(ctx, keys, vals, off, descending=False, threads=8):
 
 def boundary_offsets():
     sizes = [0, 1, 2, 3, 63, 64, 65, 255, 256, 257, 511, 512, 513, 1023, 1024, 1025,
-             2047, 2048, 2049, 4095, 4096, 4097, 8191, 8192, 8193, 16383, 16384, 16385,
+             2047, 2048, 2049, 4095, 4096, 4097, 5119, 5120, 5121, 8191, 8192, 8193,
+             16383, 16384, 16385,
              20000, 40961, 100000, 262144, 300001]
     off = np.zeros(len(sizes) + 1, dtype=np.uint64)
     np.cumsum(np.array(sizes, dtype=np.uint64), out=off[1:])
@@ -141,7 +142,7 @@ def test_tuned_table_and_rerun(ctx):
     say("segmented autotune u32 pairs")
     ctx.segmented_sort_tune(clc_hip.U32, True)
     t = ctx.segmented_sort_table(clc_hip.U32, True)
-    assert len(t) == 32 and all(0 <= c <= 7 for c in t)
+    assert len(t) == 32 and all(0 <= c <= 7 for c in t)  # 7: the onesweep path
     off = boundary_offsets()
     n = int(off[-1])
     keys = oracle.pcg32_u32(6, 6, n)
