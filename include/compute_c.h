@@ -13,9 +13,11 @@
  *     CLC_ERROR_OPENCL, which this library reuses for HIP runtime errors.
  *   - 64-bit sort keys (I64/U64/F64) are accepted (the reference returns
  *     CLC_ERROR_INVALID_ARGUMENT for them, CApi.cpp:522-524).
- *   - Operators are the reference's canonical ops:: expressions (and the
- *     projections "a", "b", plus "max(a, b)" / "min(a, b)"); any other
- *     expression returns CLC_ERROR_UNSUPPORTED instead of being compiled.
+ *   - Operator and identity strings are C++ expressions over a and b of the
+ *     element type T with OpenCL C's scalar builtins and limit macros; the
+ *     canonical ops:: forms run ahead-of-time kernels, any other expression
+ *     is compiled once per context with hipRTC (CLC_ERROR_BUILD with the
+ *     build log when it does not compile).
  *   - clc_histogram_even, clc_segmented_sort and clc_learned_sort have no C
  *     entry point in the reference.
  *
@@ -101,8 +103,9 @@ clc_status clc_device_get_info(size_t index, clc_device_info* info);
  * (reference clc_context_create, compute_c.h:89, CApi.cpp:246-249) */
 clc_status clc_context_create(size_t index, clc_context* out);
 /* Same; cache_dir (an existing directory, or NULL/"" for the environment
- * variable CL_COMPUTE_CACHE_DIR) persists the autotuner's launch
- * configurations instead of compiled programs.
+ * variable CL_COMPUTE_CACHE_DIR) persists the run-time compiled operator
+ * programs of clc_scan / clc_reduce (code objects) and the autotuner's
+ * launch configurations.
  * (reference clc_context_create_with_cache, compute_c.h:92-94,
  * CApi.cpp:251-283) */
 clc_status clc_context_create_with_cache(size_t index, const char* cache_dir,
@@ -137,6 +140,17 @@ clc_status clc_context_memory_stats(clc_context ctx, clc_memory_stats* stats);
  * MemoryResource.hpp:97-98). Synchronizes the context's stream first. */
 clc_status clc_context_release_cache(clc_context ctx);
 
+/* Counters of the context's cache of run-time compiled operator programs
+ * (the reference's program_cache::statistics, ProgramCache.hpp:57-62):
+ * programs found loaded, loaded from the cache directory, and compiled. */
+typedef struct clc_program_stats
+{
+    size_t memory_hits;
+    size_t disk_hits;
+    size_t builds;
+} clc_program_stats;
+clc_status clc_context_program_stats(clc_context ctx, clc_program_stats* stats);
+
 /* ---- kernel timing ------------------------------------------------------ */
 /* Per-kernel hipEvent timing on the context's stream, off by default (the
  * reference's samples time kernels through OpenCL event profiling,
@@ -161,7 +175,12 @@ clc_status clc_context_reset_timings(clc_context ctx);
  * one host element of the given type. */
 
 /* result = op(init, in[0] op ... op in[n-1]); n == 0 gives init. op must be
- * associative and commutative. Blocks until the result is on the host.
+ * associative and commutative, identity its identity element. op and
+ * identity are expressions as in the reference's binary_op
+ * (Algorithm.hpp:27-31): the canonical ops:: forms run ahead-of-time
+ * kernels; any other is compiled at run time (once per context, and kept in
+ * the cache directory when the context has one). Blocks until the result is
+ * on the host.
  * (reference clc_reduce, compute_c.h:156-159, CApi.cpp:486-499) */
 clc_status clc_reduce(clc_context ctx, const void* in, size_t n, clc_type type,
                       const char* op, const char* identity, const void* init,
@@ -315,9 +334,10 @@ clc_status clc_learned_segmented_sort(clc_context ctx, void* keys, clc_type key_
                                       const uint64_t* offsets, size_t num_segments,
                                       int descending, clc_learned_batched_stats* stats);
 
-/* Whether clc_scan / clc_reduce support op/identity for type:
- * CLC_SUCCESS, CLC_ERROR_UNSUPPORTED (expression not recognised) or
- * CLC_ERROR_INVALID_ARGUMENT. Needs no device. */
+/* Whether clc_scan / clc_reduce accept op/identity for type: CLC_SUCCESS
+ * (a canonical form, or an expression that compiles for gfx950),
+ * CLC_ERROR_BUILD (it does not compile; clc_last_error() has the build log)
+ * or CLC_ERROR_INVALID_ARGUMENT. Needs no device. */
 clc_status clc_operator_supported(clc_type type, const char* op,
                                   const char* identity);
 

@@ -31,9 +31,45 @@ COMMON = [
 ]
 
 
+# Headers compiled again at run time by hipRTC (operator expressions of
+# clc_scan / clc_reduce); their text is embedded in the library.
+RTC_HEADERS = ["device_common.hpp", "scan_kernels.hpp"]
+RTC_DELIM = "CLCHDR"  # raw-string delimiters hold at most 16 characters
+
+
 def sources() -> list[str]:
     return sorted(glob.glob(os.path.join(CSRC, "*.hip"))
                   + glob.glob(os.path.join(CSRC, "*.cpp")))
+
+
+def embedded_headers_source() -> str:
+    """Writes _build/rtc_embedded.cpp, defining the header names and texts
+    rtc.cpp hands to hipRTC, and returns its path (rewritten only when the
+    text changes, so incremental builds stay incremental)."""
+    names, texts = [], []
+    for h in RTC_HEADERS:
+        with open(os.path.join(CSRC, h)) as f:
+            text = f.read()
+        if f"){RTC_DELIM}\"" in text:
+            raise RuntimeError(f"{h} contains the raw-string delimiter")
+        names.append(f'"{h}"')
+        texts.append(f'R"{RTC_DELIM}({text}){RTC_DELIM}"')
+    body = ("// Generated by build.py: headers for run-time compilation.\n"
+            "namespace clc {\nnamespace rtc {\n"
+            f"extern const int embedded_header_count = {len(names)};\n"
+            "extern const char* const embedded_header_names[] = {"
+            + ", ".join(names) + "};\n"
+            "extern const char* const embedded_header_texts[] = {\n"
+            + ",\n".join(texts) + "};\n} // namespace rtc\n} // namespace clc\n")
+    path = os.path.join(BUILD, "rtc_embedded.cpp")
+    old = None
+    if os.path.exists(path):
+        with open(path) as f:
+            old = f.read()
+    if old != body:
+        with open(path, "w") as f:
+            f.write(body)
+    return path
 
 
 def _compile(src: str, extra: list[str]) -> str:
@@ -58,7 +94,7 @@ def build(jobs: int = 8, verbose: bool = False, incremental: bool = False) -> st
     object or than any header under csrc/) and links the library."""
     os.makedirs(BUILD, exist_ok=True)
     extra: list[str] = []
-    srcs = sources()
+    srcs = sources() + [embedded_headers_source()]
     headers = glob.glob(os.path.join(CSRC, "*.hpp")) + glob.glob(os.path.join(CSRC, "*.h"))
     newest = max((os.path.getmtime(h) for h in headers), default=0.0)
 
@@ -70,7 +106,8 @@ def build(jobs: int = 8, verbose: bool = False, incremental: bool = False) -> st
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         objs = list(ex.map(one, srcs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs,
+           "-L/opt/rocm/lib", "-lhiprtc", "-Wl,-rpath,/opt/rocm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")

@@ -50,6 +50,10 @@ class MemoryStats(ctypes.Structure):
                  "current_allocations", "total_allocations", "cached_bytes")]
 
 
+class ProgramStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_size_t) for n in ("memory_hits", "disk_hits", "builds")]
+
+
 class KernelTiming(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 64), ("launches", ctypes.c_size_t),
                 ("total_ms", ctypes.c_double), ("min_ms", ctypes.c_double),
@@ -85,6 +89,7 @@ SIGNATURES = {
     "clc_context_get_stream": (_i, [_vp, ctypes.POINTER(_vp)]),
     "clc_context_memory_stats": (_i, [_vp, ctypes.POINTER(MemoryStats)]),
     "clc_context_release_cache": (_i, [_vp]),
+    "clc_context_program_stats": (_i, [_vp, ctypes.POINTER(ProgramStats)]),
     "clc_context_set_profiling": (_i, [_vp, _i]),
     "clc_context_kernel_timings": (_i, [_vp, ctypes.POINTER(KernelTiming), _sz,
                                         ctypes.POINTER(_sz)]),
@@ -170,16 +175,25 @@ class Graph:
 
 
 class Context:
-    """A clc_context bound to a torch stream (default: the current one)."""
+    """A clc_context bound to a torch stream (default: the current one).
 
-    def __init__(self, stream=None):
-        import torch
-        if stream is None:
-            stream = torch.cuda.current_stream()
-        self.stream = stream
+    With cache_dir, the context is created on device 0 with its own stream
+    (clc_context_create_with_cache) and keeps run-time compiled programs and
+    tuned tables in cache_dir; callers order its work against torch's with
+    torch.cuda.synchronize() and finish()."""
+
+    def __init__(self, stream=None, cache_dir=None):
         h = ctypes.c_void_p()
-        check(lib().clc_context_create_from_stream(ctypes.c_void_p(stream.cuda_stream),
-                                                   ctypes.byref(h)))
+        if cache_dir is not None:
+            self.stream = None
+            check(lib().clc_context_create_with_cache(0, _enc(cache_dir), ctypes.byref(h)))
+        else:
+            import torch
+            if stream is None:
+                stream = torch.cuda.current_stream()
+            self.stream = stream
+            check(lib().clc_context_create_from_stream(
+                ctypes.c_void_p(stream.cuda_stream), ctypes.byref(h)))
         self.handle = h
 
     def close(self):
@@ -195,6 +209,11 @@ class Context:
 
     def finish(self):
         check(lib().clc_finish(self.handle))
+
+    def program_stats(self) -> ProgramStats:
+        s = ProgramStats()
+        check(lib().clc_context_program_stats(self.handle, ctypes.byref(s)))
+        return s
 
     def memory_stats(self) -> MemoryStats:
         s = MemoryStats()

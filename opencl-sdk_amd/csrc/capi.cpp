@@ -15,6 +15,7 @@
 #include "context.hpp"
 #include "learned_sort.hpp"
 #include "ops.hpp"
+#include "rtc.hpp"
 #include "segmented_sort.hpp"
 
 struct clc_context_s
@@ -53,6 +54,9 @@ template <typename F> clc_status guarded(F&& body)
     } catch (const clc::unsupported_error& e)
     {
         return fail(CLC_ERROR_UNSUPPORTED, e.what());
+    } catch (const clc::build_error& e)
+    {
+        return fail(CLC_ERROR_BUILD, e.what());
     } catch (const clc::out_of_memory& e)
     {
         return fail(CLC_ERROR_OUT_OF_MEMORY, e.what(),
@@ -93,6 +97,30 @@ std::mutex& init_mutex()
 void require(bool condition, const char* message)
 {
     if (!condition) throw std::invalid_argument(message);
+}
+
+// True when op and identity are forms the ahead-of-time kernels implement
+// (code and id filled in). Anything else -- another expression, an
+// identity that is not a constant the parser knows, a bitwise operator on
+// floating-point elements -- is compiled at run time, where OpenCL C's
+// rules decide, as in the reference (a bitwise float operator is a build
+// error there and here).
+bool canonical(int type, const char* op, const char* identity, clc::op_code& code,
+               unsigned char* id)
+{
+    clc::type_size(type); // rejects unknown types
+    try
+    {
+        code = clc::parse_op(op, type);
+        clc::parse_identity(identity, type, id);
+        return true;
+    } catch (const clc::unsupported_error&)
+    {
+        return false;
+    } catch (const std::invalid_argument&)
+    {
+        return false;
+    }
 }
 
 std::size_t device_count_locked()
@@ -253,6 +281,17 @@ clc_status clc_context_memory_stats(clc_context ctx, clc_memory_stats* stats)
     });
 }
 
+clc_status clc_context_program_stats(clc_context ctx, clc_program_stats* stats)
+{
+    return guarded([&] {
+        require(ctx != nullptr && stats != nullptr, "clc: null argument");
+        const clc::rtc::cache_stats& c = get(ctx).programs().stats();
+        stats->memory_hits = c.memory_hits;
+        stats->disk_hits = c.disk_hits;
+        stats->builds = c.builds;
+    });
+}
+
 clc_status clc_context_release_cache(clc_context ctx)
 {
     if (ctx == nullptr)
@@ -313,10 +352,12 @@ clc_status clc_reduce(clc_context ctx, const void* in, size_t n, clc_type type,
         require(ctx != nullptr && op != nullptr && identity != nullptr
                     && init != nullptr && result != nullptr,
                 "clc: null argument");
-        const clc::op_code code = clc::parse_op(op, type);
+        clc::op_code code;
         unsigned char id[8];
-        clc::parse_identity(identity, type, id);
-        clc::detail::reduce(get(ctx), in, n, type, code, id, init, result);
+        if (canonical(type, op, identity, code, id))
+            clc::detail::reduce(get(ctx), in, n, type, code, id, init, result);
+        else
+            clc::detail::reduce_expr(get(ctx), in, n, type, op, identity, init, result);
     });
 }
 
@@ -327,10 +368,13 @@ clc_status clc_scan(clc_context ctx, const void* in, void* out, size_t n,
     return guarded([&] {
         require(ctx != nullptr && op != nullptr && identity != nullptr,
                 "clc: null argument");
-        const clc::op_code code = clc::parse_op(op, type);
+        clc::op_code code;
         unsigned char id[8];
-        clc::parse_identity(identity, type, id);
-        clc::detail::scan(get(ctx), in, out, n, type, code, id, exclusive != 0);
+        if (canonical(type, op, identity, code, id))
+            clc::detail::scan(get(ctx), in, out, n, type, code, id, exclusive != 0);
+        else
+            clc::detail::scan_expr(get(ctx), in, out, n, type, op, identity,
+                                   exclusive != 0);
     });
 }
 
@@ -475,10 +519,11 @@ clc_status clc_operator_supported(clc_type type, const char* op,
 {
     return guarded([&] {
         require(op != nullptr && identity != nullptr, "clc: null argument");
-        clc::type_size(type);
-        clc::parse_op(op, type);
+        clc::op_code code;
         unsigned char id[8];
-        clc::parse_identity(identity, type, id);
+        if (!canonical(type, op, identity, code, id))
+            (void)clc::rtc::compile(clc::rtc::operator_source(type, op, identity, false),
+                                    "gfx950");
     });
 }
 

@@ -1,4 +1,5 @@
 #include "context.hpp"
+#include "rtc.hpp"
 
 #include <algorithm>
 #include <cstdio>
@@ -70,6 +71,7 @@ context::~context()
 {
     (void)hipSetDevice(device_);
     (void)hipStreamSynchronize(stream_);
+    programs_.reset(); // unloads run-time compiled modules
     if (pinned_) (void)hipHostFree(pinned_);
     resource_.reset();
     pool_.reset(); // returns cached blocks
@@ -193,6 +195,12 @@ void store_tuning(const context& ctx, const std::string& name,
         if (!f) return;
     }
     if (std::rename(tmp.c_str(), path.c_str()) != 0) std::remove(tmp.c_str());
+}
+
+rtc::program_cache& context::programs()
+{
+    if (!programs_) programs_.reset(new rtc::program_cache(cache_dir_, facts_.arch));
+    return *programs_;
 }
 
 void context::activate() const

@@ -17,6 +17,10 @@
 
 namespace clc {
 
+namespace rtc {
+class program_cache;
+}
+
 //! A failed HIP runtime call (CLC_ERROR_OPENCL at the C boundary).
 struct hip_error : std::runtime_error
 {
@@ -126,6 +130,9 @@ public:
     //! Autotuned launch tables, keyed by the tuner (key type, values, ...).
     std::map<int, std::vector<int>>& tuned() { return tuned_; }
 
+    //! Run-time compiled operator programs (created on first use).
+    rtc::program_cache& programs();
+
     //! Temporarily redirects launches and scratch (used while a graph is
     //! captured on a private stream with graph-owned scratch); returns the
     //! previous values so the caller can restore them.
@@ -154,6 +161,7 @@ private:
     void* pinned_ = nullptr;
     class profiler profiler_;
     std::map<int, std::vector<int>> tuned_;
+    std::unique_ptr<rtc::program_cache> programs_;
 };
 
 //! Brackets one kernel launch with profiler events when profiling is on.
@@ -207,6 +215,13 @@ void scan(context& ctx, const void* in, void* out, std::size_t n, int type,
           int op, const void* identity, bool exclusive);
 void reduce(context& ctx, const void* in, std::size_t n, int type, int op,
             const void* identity, const void* init, void* result);
+//! scan / reduce with an operator and identity given as OpenCL C
+//! expressions, compiled at run time (the context's program cache).
+void scan_expr(context& ctx, const void* in, void* out, std::size_t n, int type,
+               const char* op, const char* identity, bool exclusive);
+void reduce_expr(context& ctx, const void* in, std::size_t n, int type,
+                 const char* op, const char* identity, const void* init,
+                 void* result);
 void histogram_even(context& ctx, const void* in, std::size_t n, int type,
                     uint32_t* histogram, std::size_t num_bins, double lower,
                     double upper);

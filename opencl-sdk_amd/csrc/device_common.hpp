@@ -1,11 +1,16 @@
 // Device helpers shared by the gfx950 kernels: wave64 lane arithmetic,
 // agent-scope atomics for cross-workgroup status words, and the
 // order-preserving key images of the reference sort.
+//
+// Also compiled at run time by hipRTC (through scan_kernels.hpp), which
+// provides the HIP runtime implicitly and has no C library headers.
 #pragma once
 
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#endif
 #include <cstdint>
 #include <type_traits>
 

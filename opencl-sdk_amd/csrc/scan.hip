@@ -17,8 +17,9 @@
 // on the host. A grid of partial folds, then one workgroup folds the
 // partials and applies init.
 #include "context.hpp"
-#include "device_common.hpp"
 #include "ops.hpp"
+#include "rtc.hpp"
+#include "scan_kernels.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -36,250 +37,30 @@ template <> struct wrap_type<int64_t> { using type = uint64_t; };
 
 //! The canonical operators (reference Algorithm.hpp:53-65) plus the
 //! projections. Integer plus/multiply wrap, as in OpenCL C.
-template <typename T, int OP>
-__device__ __forceinline__ T apply(T a, T b)
+template <typename T, int OP> struct canonical_op
 {
-    using U = typename wrap_type<T>::type;
-    if constexpr (OP == OP_PLUS) return static_cast<T>(static_cast<U>(a) + static_cast<U>(b));
-    else if constexpr (OP == OP_MULTIPLIES) return static_cast<T>(static_cast<U>(a) * static_cast<U>(b));
-    else if constexpr (OP == OP_BIT_AND) return static_cast<T>(a & b);
-    else if constexpr (OP == OP_BIT_OR) return static_cast<T>(a | b);
-    else if constexpr (OP == OP_BIT_XOR) return static_cast<T>(a ^ b);
-    else if constexpr (OP == OP_MINIMUM) return (b < a) ? b : a;
-    else if constexpr (OP == OP_MAXIMUM) return (a < b) ? b : a;
-    else if constexpr (OP == OP_RIGHT) return b;
-    else return a;
-}
-
-template <typename T> __device__ __forceinline__ T shfl_up(T x, int o)
-{
-    if constexpr (sizeof(T) <= 4)
+    static __device__ __forceinline__ T apply(T a, T b)
     {
-        uint32_t u = 0;
-        __builtin_memcpy(&u, &x, sizeof(T));
-        u = __shfl_up(u, o, WAVE);
-        T r;
-        __builtin_memcpy(&r, &u, sizeof(T));
-        return r;
+        using U = typename wrap_type<T>::type;
+        if constexpr (OP == OP_PLUS) return static_cast<T>(static_cast<U>(a) + static_cast<U>(b));
+        else if constexpr (OP == OP_MULTIPLIES) return static_cast<T>(static_cast<U>(a) * static_cast<U>(b));
+        else if constexpr (OP == OP_BIT_AND) return static_cast<T>(a & b);
+        else if constexpr (OP == OP_BIT_OR) return static_cast<T>(a | b);
+        else if constexpr (OP == OP_BIT_XOR) return static_cast<T>(a ^ b);
+        else if constexpr (OP == OP_MINIMUM) return (b < a) ? b : a;
+        else if constexpr (OP == OP_MAXIMUM) return (a < b) ? b : a;
+        else if constexpr (OP == OP_RIGHT) return b;
+        else return a;
     }
-    else
-    {
-        uint64_t u;
-        __builtin_memcpy(&u, &x, 8);
-        uint32_t lo = __shfl_up(static_cast<uint32_t>(u), o, WAVE);
-        uint32_t hi = __shfl_up(static_cast<uint32_t>(u >> 32), o, WAVE);
-        u = (static_cast<uint64_t>(hi) << 32) | lo;
-        T r;
-        __builtin_memcpy(&r, &u, 8);
-        return r;
-    }
-}
-
-template <typename T> __device__ __forceinline__ T shfl_down(T x, int o)
-{
-    if constexpr (sizeof(T) <= 4)
-    {
-        uint32_t u = 0;
-        __builtin_memcpy(&u, &x, sizeof(T));
-        u = __shfl_down(u, o, WAVE);
-        T r;
-        __builtin_memcpy(&r, &u, sizeof(T));
-        return r;
-    }
-    else
-    {
-        uint64_t u;
-        __builtin_memcpy(&u, &x, 8);
-        uint32_t lo = __shfl_down(static_cast<uint32_t>(u), o, WAVE);
-        uint32_t hi = __shfl_down(static_cast<uint32_t>(u >> 32), o, WAVE);
-        u = (static_cast<uint64_t>(hi) << 32) | lo;
-        T r;
-        __builtin_memcpy(&r, &u, 8);
-        return r;
-    }
-}
-
-//! Inclusive wave scan in lane order (lower lanes are left operands).
-template <typename T, int OP> __device__ __forceinline__ T wave_scan(T x)
-{
-    const uint32_t lane = lane_id();
-#pragma unroll
-    for (int o = 1; o < WAVE; o <<= 1)
-    {
-        const T y = shfl_up(x, o);
-        if (lane >= static_cast<uint32_t>(o)) x = apply<T, OP>(y, x);
-    }
-    return x;
-}
-
-// ---- scan status words -------------------------------------------------------
-// A tile's published value is up to 8 bytes; it is split into two 64-bit
-// words, each carrying the flag in its top half, so each word is read and
-// written atomically and a reader accepts the pair only when both carry the
-// same flag (flags only move 0 -> AGG -> PREFIX).
-
-constexpr uint64_t FLAG_AGG = uint64_t(1) << 32, FLAG_PREFIX = uint64_t(2) << 32;
-
-template <typename T>
-__device__ __forceinline__ void publish(uint64_t* st, T v, uint64_t flag)
-{
-    uint64_t bits = 0;
-    __builtin_memcpy(&bits, &v, sizeof(T));
-    store_agent(st + 1, flag | (bits >> 32));
-    store_agent(st, flag | (bits & uint64_t(0xFFFFFFFFu)));
-}
+};
 
 template <typename T, int OP, int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void scan_kernel(
     const T* in, T* out, size_t n, T identity, int exclusive, int vec_ok,
     uint64_t* status, uint32_t* tile_counter)
 {
-    constexpr int W = BLOCK / WAVE;
-    constexpr size_t TILE = size_t(BLOCK) * ITEMS;
-    constexpr int VEC = 16 / sizeof(T); // elements per 16-byte load
-    static_assert(ITEMS % VEC == 0, "items per thread");
-    __shared__ T s_wave[W];
-    __shared__ T s_prefix;
-    __shared__ uint32_t s_tile;
-
-    const uint32_t tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
-    if (tid == 0) s_tile = fetch_add_agent(tile_counter, 1u);
-    __syncthreads();
-    const size_t tile = s_tile;
-    const size_t first = tile * TILE + size_t(tid) * ITEMS;
-    const bool full = (tile + 1) * TILE <= n && vec_ok;
-
-    T x[ITEMS];
-    if (full)
-    {
-        const uint4* src = reinterpret_cast<const uint4*>(in + first);
-#pragma unroll
-        for (int j = 0; j < ITEMS / VEC; ++j)
-        {
-            const uint4 q = src[j];
-            __builtin_memcpy(&x[j * VEC], &q, 16);
-        }
-    }
-    else
-    {
-#pragma unroll
-        for (int j = 0; j < ITEMS; ++j)
-            x[j] = first + j < n ? in[first + j] : identity;
-    }
-
-    // Prefixes are computed unseeded (no identity inside), so operators
-    // without a two-sided identity (the projections) still give the
-    // sequential result; the identity is applied once, on output.
-    // Thread fold, then wave and block exclusive scans of the folds.
-    T agg = x[0];
-#pragma unroll
-    for (int j = 1; j < ITEMS; ++j) agg = apply<T, OP>(agg, x[j]);
-    const T inc = wave_scan<T, OP>(agg);
-    if (lane == WAVE - 1) s_wave[wave] = inc;
-    const T excl_lane = shfl_up(inc, 1); // meaningful for lane > 0
-    __syncthreads();
-    T wave_prefix = s_wave[0]; // meaningful for wave > 0
-    for (int w = 1; w < W; ++w)
-        if (w < static_cast<int>(wave)) wave_prefix = apply<T, OP>(wave_prefix, s_wave[w]);
-    T block_agg = s_wave[0];
-    for (int w = 1; w < W; ++w) block_agg = apply<T, OP>(block_agg, s_wave[w]);
-
-    // Look-back by thread 0.
-    if (tid == 0)
-    {
-        uint64_t* mine = status + 2 * tile;
-        if (tile == 0)
-        {
-            publish(mine, block_agg, FLAG_PREFIX);
-            s_prefix = identity;
-        }
-        else
-        {
-            publish(mine, block_agg, FLAG_AGG);
-            T acc = identity;
-            bool got = false;
-            size_t p = tile - 1;
-            for (;;)
-            {
-                const uint64_t lo = load_agent(status + 2 * p);
-                const uint64_t hi = load_agent(status + 2 * p + 1);
-                const uint64_t f = lo & ~uint64_t(0xFFFFFFFFu);
-                if (f == 0 || f != (hi & ~uint64_t(0xFFFFFFFFu))) continue;
-                const uint64_t bits = (lo & uint64_t(0xFFFFFFFFu)) | (hi << 32);
-                T val;
-                __builtin_memcpy(&val, &bits, sizeof(T));
-                acc = got ? apply<T, OP>(val, acc) : val;
-                got = true;
-                if (f == FLAG_PREFIX) break;
-                --p;
-            }
-            publish(mine, apply<T, OP>(acc, block_agg), FLAG_PREFIX);
-            s_prefix = acc;
-        }
-    }
-    __syncthreads();
-
-    // pre = tile prefix op wave prefix op lane prefix, each when present.
-    T pre = s_prefix;
-    bool has = tile > 0;
-    if (wave > 0)
-    {
-        pre = has ? apply<T, OP>(pre, wave_prefix) : wave_prefix;
-        has = true;
-    }
-    if (lane > 0)
-    {
-        pre = has ? apply<T, OP>(pre, excl_lane) : excl_lane;
-        has = true;
-    }
-    // out = identity op (in[0] op ... op in[i]) (inclusive) or
-    //       identity op (in[0] op ... op in[i-1]) (exclusive; identity at 0).
-    T y[ITEMS];
-#pragma unroll
-    for (int j = 0; j < ITEMS; ++j)
-    {
-        const T next = has ? apply<T, OP>(pre, x[j]) : x[j];
-        y[j] = exclusive ? (has ? apply<T, OP>(identity, pre) : identity)
-                         : apply<T, OP>(identity, next);
-        pre = next;
-        has = true;
-    }
-    if (full)
-    {
-        uint4* dst = reinterpret_cast<uint4*>(out + first);
-#pragma unroll
-        for (int j = 0; j < ITEMS / VEC; ++j)
-        {
-            uint4 q;
-            __builtin_memcpy(&q, &y[j * VEC], 16);
-            dst[j] = q;
-        }
-    }
-    else
-    {
-#pragma unroll
-        for (int j = 0; j < ITEMS; ++j)
-            if (first + j < n) out[first + j] = y[j];
-    }
-}
-
-// ---- reduce --------------------------------------------------------------------
-
-template <typename T, int OP>
-__device__ __forceinline__ T block_reduce(T x, T* s_wave)
-{
-    const uint32_t tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
-#pragma unroll
-    for (int o = WAVE / 2; o > 0; o >>= 1)
-    {
-        const T y = shfl_down(x, o);
-        if (lane + o < WAVE) x = apply<T, OP>(x, y);
-    }
-    if (lane == 0) s_wave[wave] = x;
-    __syncthreads();
-    T r = s_wave[0];
-    for (uint32_t w = 1; w < blockDim.x / WAVE; ++w) r = apply<T, OP>(r, s_wave[w]);
-    return r;
+    scan_tile<T, canonical_op<T, OP>, BLOCK, ITEMS>(in, out, n, identity, exclusive,
+                                                   vec_ok, status, tile_counter);
 }
 
 template <typename T, int OP, int BLOCK>
@@ -287,29 +68,8 @@ __global__ __launch_bounds__(BLOCK) void reduce_kernel(
     const T* __restrict__ in, size_t n, T identity, int vec_ok,
     T* __restrict__ out, int apply_init, T init)
 {
-    __shared__ T s_wave[BLOCK / WAVE];
-    constexpr int VEC = 16 / sizeof(T);
-    T acc = identity;
-    const size_t stride = size_t(gridDim.x) * BLOCK;
-    size_t start = 0;
-    if (vec_ok)
-    {
-        const size_t nvec = n / VEC;
-        const uint4* v = reinterpret_cast<const uint4*>(in);
-        for (size_t i = size_t(blockIdx.x) * BLOCK + threadIdx.x; i < nvec; i += stride)
-        {
-            const uint4 q = v[i];
-            T e[VEC];
-            __builtin_memcpy(e, &q, 16);
-#pragma unroll
-            for (int j = 0; j < VEC; ++j) acc = apply<T, OP>(acc, e[j]);
-        }
-        start = nvec * VEC;
-    }
-    for (size_t i = start + size_t(blockIdx.x) * BLOCK + threadIdx.x; i < n; i += stride)
-        acc = apply<T, OP>(acc, in[i]);
-    const T r = block_reduce<T, OP>(acc, s_wave);
-    if (threadIdx.x == 0) out[blockIdx.x] = apply_init ? apply<T, OP>(init, r) : r;
+    reduce_partial<T, canonical_op<T, OP>, BLOCK>(in, n, identity, vec_ok, out,
+                                                 apply_init, init);
 }
 
 // ---- host side -------------------------------------------------------------------
@@ -318,8 +78,8 @@ template <typename T, int OP>
 void scan_typed(context& ctx, const void* in, void* out, size_t n,
                 const void* identity, bool exclusive)
 {
-    constexpr int BLOCK = 256;
-    constexpr int ITEMS = sizeof(T) >= 4 ? 64 / sizeof(T) : 16;
+    constexpr int BLOCK = SCAN_BLOCK;
+    constexpr int ITEMS = scan_items<T>();
     constexpr size_t TILE = size_t(BLOCK) * ITEMS;
     const size_t tiles = (n + TILE - 1) / TILE;
     T id;
@@ -343,7 +103,7 @@ template <typename T, int OP>
 void reduce_typed(context& ctx, const void* in, size_t n, const void* identity,
                   const void* init, void* result)
 {
-    constexpr int BLOCK = 256;
+    constexpr int BLOCK = SCAN_BLOCK;
     T id, ini;
     std::memcpy(&id, identity, sizeof(T));
     std::memcpy(&ini, init, sizeof(T));
@@ -464,6 +224,90 @@ void reduce(context& ctx, const void* in, std::size_t n, int type, int op,
     if (in == nullptr) throw std::invalid_argument("clc_reduce: null buffer");
     ctx.activate();
     by_type<reduce_fn>(type, op, ctx, in, n, identity, init, result);
+}
+
+void scan_expr(context& ctx, const void* in, void* out, std::size_t n, int type,
+               const char* op, const char* identity, bool exclusive)
+{
+    const std::size_t es = type_size(type);
+    if (op == nullptr || identity == nullptr)
+        throw std::invalid_argument("clc_scan: null operator or identity");
+    if (n == 0) return;
+    check_size(n);
+    if (in == nullptr || out == nullptr)
+        throw std::invalid_argument("clc_scan: null buffer");
+    ctx.activate();
+    hipFunction_t f = ctx.programs().function(
+        rtc::operator_source(type, op, identity, true), rtc::SCAN_KERNEL);
+    // Same tiling as the ahead-of-time scan (scan_items<T>() elements a
+    // thread, SCAN_BLOCK threads a tile).
+    const std::size_t items = es >= 4 ? 64 / es : 16;
+    const std::size_t tile = std::size_t(SCAN_BLOCK) * items;
+    const std::size_t tiles = (n + tile - 1) / tile;
+    int vec_ok = (reinterpret_cast<uintptr_t>(in) % 16 == 0
+                  && reinterpret_cast<uintptr_t>(out) % 16 == 0)
+        ? 1 : 0;
+    int ex = exclusive ? 1 : 0;
+    const std::size_t bytes = 256 + tiles * 16;
+    temp_buffer tmp(ctx.resource(), bytes);
+    char* t = tmp.as<char>();
+    detail::zero_fill(ctx.stream(), t, bytes);
+    void* status = t + 256;
+    void* counter = t;
+    void* args[] = { &in, &out, &n, &ex, &vec_ok, &status, &counter };
+    timed_launch tl(ctx, "rtc_scan");
+    CLC_HIP(hipModuleLaunchKernel(f, static_cast<unsigned>(tiles), 1, 1, SCAN_BLOCK, 1, 1,
+                                  0, ctx.stream(), args, nullptr));
+}
+
+void reduce_expr(context& ctx, const void* in, std::size_t n, int type,
+                 const char* op, const char* identity, const void* init,
+                 void* result)
+{
+    const std::size_t es = type_size(type);
+    if (op == nullptr || identity == nullptr)
+        throw std::invalid_argument("clc_reduce: null operator or identity");
+    if (n == 0)
+    {
+        std::memcpy(result, init, es);
+        return;
+    }
+    check_size(n);
+    if (in == nullptr) throw std::invalid_argument("clc_reduce: null buffer");
+    ctx.activate();
+    hipFunction_t f = ctx.programs().function(
+        rtc::operator_source(type, op, identity, false), rtc::REDUCE_KERNEL);
+    const std::size_t per_block = std::size_t(SCAN_BLOCK) * 16;
+    const uint32_t groups = static_cast<uint32_t>(std::max<std::size_t>(
+        1, std::min<std::size_t>((n + per_block - 1) / per_block,
+                                 std::size_t(ctx.facts().cu_count) * 4)));
+    temp_buffer partials(ctx.resource(), (groups + 1) * es);
+    char* part = partials.as<char>();
+    unsigned char ini[8] = {};
+    std::memcpy(ini, init, es);
+    {
+        int vec_ok = reinterpret_cast<uintptr_t>(in) % 16 == 0 ? 1 : 0;
+        int apply = 0;
+        void* out = part;
+        void* args[] = { &in, &n, &vec_ok, &out, &apply, ini };
+        timed_launch tl(ctx, "rtc_reduce");
+        CLC_HIP(hipModuleLaunchKernel(f, groups, 1, 1, SCAN_BLOCK, 1, 1, 0,
+                                      ctx.stream(), args, nullptr));
+    }
+    {
+        const void* src = part;
+        std::size_t count = groups;
+        int vec_ok = 0, apply = 1;
+        void* out = part + std::size_t(groups) * es;
+        void* args[] = { &src, &count, &vec_ok, &out, &apply, ini };
+        timed_launch tl(ctx, "rtc_reduce");
+        CLC_HIP(hipModuleLaunchKernel(f, 1, 1, 1, SCAN_BLOCK, 1, 1, 0, ctx.stream(),
+                                      args, nullptr));
+    }
+    CLC_HIP(hipMemcpyAsync(ctx.pinned(), part + std::size_t(groups) * es, es,
+                           hipMemcpyDeviceToHost, ctx.stream()));
+    CLC_HIP(hipStreamSynchronize(ctx.stream()));
+    std::memcpy(result, ctx.pinned(), es);
 }
 
 } // namespace detail

@@ -219,6 +219,54 @@ def reduce(x: np.ndarray, op: str = "plus", identity=0, init=0):
     return res[0]
 
 
+def _element_converter(dtype):
+    """Python number -> the value C's conversion to dtype gives: integers
+    wrap modulo 2^bits (two's complement), floats round to the type."""
+    dt = np.dtype(dtype)
+    if dt.kind in "iu":
+        bits = dt.itemsize * 8
+        mask = (1 << bits) - 1
+        signed = dt.kind == "i"
+
+        def conv(v):
+            v = int(v) & mask
+            return v - (1 << bits) if signed and v >> (bits - 1) else v
+        return conv
+    if dt == np.float32:
+        return lambda v: float(np.float32(v))
+    return float
+
+
+def scan_fn(x: np.ndarray, fn, identity, exclusive: bool = False) -> np.ndarray:
+    """The reference's scan (running = IDENTITY; next = OP(running, x[i]);
+    out[i] = running if exclusive else next -- Algorithm.cpp:85-104) with
+    the operator given as a Python function of two Python numbers, for
+    operator expressions the library compiles at run time. Each result is
+    converted to x's dtype as C converts the expression's value (T)(OP).
+    A plain loop: small n only."""
+    conv = _element_converter(x.dtype)
+    run = conv(identity)
+    res = []
+    for v in x.tolist():
+        nxt = conv(fn(run, v))
+        res.append(run if exclusive else nxt)
+        run = nxt
+    return np.array(res, dtype=x.dtype)
+
+
+def reduce_fn(x: np.ndarray, fn, identity, init):
+    """OP(init, fold) with the fold seeded by IDENTITY (Algorithm.cpp:58-74,
+    :400-442), the operator a Python function as in scan_fn. n == 0 gives
+    init."""
+    conv = _element_converter(x.dtype)
+    if x.size == 0:
+        return conv(init)
+    acc = conv(identity)
+    for v in x.tolist():
+        acc = conv(fn(acc, v))
+    return conv(fn(conv(init), acc))
+
+
 def histogram_even(x: np.ndarray, bins: int, lower: float, upper: float) -> np.ndarray:
     """Even-width histogram with the reference's float binning
     (Algorithm.cpp:240-273, :569-615)."""

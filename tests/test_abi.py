@@ -51,7 +51,7 @@ def test_null_arguments_are_invalid_argument():
 
 def test_last_error_cleared_on_success():
     L = clc_hip.lib()
-    assert L.clc_operator_supported(clc_hip.U32, b"nonsense", b"0") == clc_hip.ERROR_UNSUPPORTED
+    assert L.clc_operator_supported(clc_hip.U32, b"nonsense", b"0") == clc_hip.ERROR_BUILD
     assert len(L.clc_last_error()) > 0
     assert L.clc_operator_supported(clc_hip.U32, b"(a + b)", b"0") == clc_hip.SUCCESS
     assert L.clc_last_error() == b""
@@ -90,12 +90,68 @@ def test_canonical_operators_supported(t, op, ident):
 
 
 @pytest.mark.parametrize("t,op,ident,status", [
-    (clc_hip.U32, "a - b", "0", clc_hip.ERROR_UNSUPPORTED),
-    (clc_hip.U32, "x +* nonsense", "0", clc_hip.ERROR_UNSUPPORTED),
-    (clc_hip.F32, "(a & b)", "0", clc_hip.ERROR_INVALID_ARGUMENT),
-    (clc_hip.U32, "(a + b)", "zero", clc_hip.ERROR_UNSUPPORTED),
-    (clc_hip.F32, "(a + b)", "1 + 2", clc_hip.ERROR_UNSUPPORTED),
+    (clc_hip.U32, "a - b", "0", clc_hip.SUCCESS),  # compiles; associativity is the caller's
+    (clc_hip.U32, "x +* nonsense", "0", clc_hip.ERROR_BUILD),
+    (clc_hip.F32, "(a & b)", "0", clc_hip.ERROR_BUILD),  # as OpenCL C: no & on float
+    (clc_hip.U32, "(a + b)", "zero", clc_hip.ERROR_BUILD),
+    (clc_hip.F32, "(a + b)", "1 + 2", clc_hip.SUCCESS),
+    (clc_hip.U32, "(a + b", "0", clc_hip.ERROR_BUILD),
     (42, "(a + b)", "0", clc_hip.ERROR_INVALID_ARGUMENT),
+    (42, "a - b", "0", clc_hip.ERROR_INVALID_ARGUMENT),
 ])
-def test_unsupported_operators(t, op, ident, status):
-    assert clc_hip.lib().clc_operator_supported(t, op.encode(), ident.encode()) == status
+def test_operator_expressions_compile_or_fail(t, op, ident, status):
+    """Non-canonical expressions are compiled at run time (no device needed
+    for the check); one that does not compile is CLC_ERROR_BUILD with the
+    build log, as the reference's capi test expects (capi.c:151-156)."""
+    L = clc_hip.lib()
+    assert L.clc_operator_supported(t, op.encode(), ident.encode()) == status, \
+        L.clc_last_error().decode()
+    if status == clc_hip.ERROR_BUILD:
+        assert b"build log" in L.clc_last_error()
+
+
+def test_build_log_names_the_expression():
+    L = clc_hip.lib()
+    assert L.clc_operator_supported(clc_hip.I32, b"a + undefined_name", b"0") \
+        == clc_hip.ERROR_BUILD
+    err = L.clc_last_error().decode()
+    assert "operator:1:" in err and "undefined_name" in err
+    assert L.clc_operator_supported(clc_hip.I32, b"a + b", b"bad_identity") \
+        == clc_hip.ERROR_BUILD
+    assert "identity:1:" in L.clc_last_error().decode()
+
+
+# OpenCL C scalar builtins, limit macros, conversions and type names that
+# operator expressions may use, on every element type they apply to.
+INT_TYPES = [clc_hip.I8, clc_hip.U8, clc_hip.I16, clc_hip.U16, clc_hip.I32,
+             clc_hip.U32, clc_hip.I64, clc_hip.U64]
+FLOAT_TYPES = [clc_hip.F32, clc_hip.F64]
+
+
+@pytest.mark.parametrize("t", INT_TYPES)
+@pytest.mark.parametrize("op,ident", [
+    ("max(a, b)", "0"), ("min(a, b) + clamp(a, 0, 100)", "0"),
+    ("abs(a) + abs_diff(a, b) + add_sat(a, b) + sub_sat(a, b)", "0"),
+    ("hadd(a, b) ^ rhadd(a, b) ^ mul_hi(a, b) ^ mad_hi(a, b, a)", "0"),
+    ("rotate(a, 3) + popcount(b) + clz(a) + mad24(a, b, a) + mul24(a, b)", "0"),
+    ("select(a, b, a < b) + mad(a, b, 1)", "CHAR_MAX + SHRT_MIN + UINT_MAX + LONG_MIN"),
+    ("(T)(convert_int(a) + convert_ulong(b) + as_uint(1.0f) + (uint)(uchar)a)", "INT_MIN"),
+])
+def test_integer_builtins_compile(t, op, ident):
+    L = clc_hip.lib()
+    assert L.clc_operator_supported(t, op.encode(), ident.encode()) == clc_hip.SUCCESS, \
+        L.clc_last_error().decode()[-2000:]
+
+
+@pytest.mark.parametrize("t", FLOAT_TYPES)
+@pytest.mark.parametrize("op,ident", [
+    ("fmax(a, b)", "-INFINITY"), ("fmin(a, b)", "INFINITY"),
+    ("max(a, b) + min(a, b) + clamp(a, -1.0f, 1.0f)", "-FLT_MAX"),
+    ("mad(a, b, 1) + mix(a, b, 0.5f) + sign(a) + select(a, b, a < b)", "0.0"),
+    ("fabs(a) + abs(b) + sqrt(a) + fma(a, b, a) + convert_float(b)", "DBL_MAX"),
+    ("a + b", "NAN"), ("a * b", "HUGE_VALF + FLT_MIN + FLT_EPSILON + MAXFLOAT"),
+])
+def test_float_builtins_compile(t, op, ident):
+    L = clc_hip.lib()
+    assert L.clc_operator_supported(t, op.encode(), ident.encode()) == clc_hip.SUCCESS, \
+        L.clc_last_error().decode()[-2000:]

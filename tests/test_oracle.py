@@ -191,6 +191,88 @@ def test_reduce_empty_returns_init_and_float_minmax():
         oracle.reduce(f, "bit_and", 0, 0)
 
 
+# ---- scan_fn / reduce_fn (operators as Python functions) ---------------------
+
+@pytest.mark.parametrize("n", SIZES)
+def test_scan_fn_reference_test_vectors(golden, n):
+    x = golden[f"scan_in_{n}"]
+    plus = lambda a, b: a + b  # noqa: E731
+    np.testing.assert_array_equal(oracle.scan_fn(x, plus, 0),
+                                  golden[f"scan_expect_inclusive_{n}"])
+    np.testing.assert_array_equal(oracle.scan_fn(x, plus, 0, exclusive=True),
+                                  golden[f"scan_expect_exclusive_{n}"])
+
+
+def test_scan_fn_projection_and_max_reference_vectors(golden):
+    v = golden["proj_in"]
+    np.testing.assert_array_equal(oracle.scan_fn(v, lambda a, b: b, 0), v)
+    np.testing.assert_array_equal(oracle.scan_fn(v, max, np.iinfo(np.int32).min),
+                                  golden["max_scan_expect"])
+
+
+FNS = {"plus": lambda a, b: a + b, "multiplies": lambda a, b: a * b,
+       "bit_and": lambda a, b: a & b, "bit_or": lambda a, b: a | b,
+       "bit_xor": lambda a, b: a ^ b, "minimum": lambda a, b: b if b < a else a,
+       "maximum": lambda a, b: b if a < b else a, "right": lambda a, b: b,
+       "left": lambda a, b: a}
+
+
+@pytest.mark.parametrize("dtype", [np.int8, np.uint8, np.int16, np.uint16, np.int32,
+                                   np.uint32, np.int64, np.uint64, np.float32,
+                                   np.float64])
+def test_scan_fn_and_reduce_fn_match_the_c_oracle(dtype):
+    """Every canonical operator, every type, wrap-around included: the
+    Python restatement equals oracle.c's (pinned by the vectors above)."""
+    dt = np.dtype(dtype)
+    rng = np.random.default_rng(dt.itemsize * 7 + (dt.kind == "f"))
+    n = 3001
+    if dt.kind == "f":
+        x = rng.integers(-50, 50, size=n).astype(dt)
+        ops = ["plus", "minimum", "maximum", "right", "left"]
+    else:
+        info = np.iinfo(dt)
+        x = rng.integers(info.min, info.max, size=n, dtype=dt, endpoint=True)
+        ops = list(FNS)
+    for op in ops + (["multiplies"] if dt.kind == "f" else []):
+        ident = 1 if op == "multiplies" else 0
+        xs = x[:40] if (op == "multiplies" and dt.kind == "f") else x
+        for ex in (False, True):
+            np.testing.assert_array_equal(oracle.scan_fn(xs, FNS[op], ident, ex),
+                                          oracle.scan(xs, op, ident, ex), err_msg=op)
+        if op not in ("right", "left"):
+            got = oracle.reduce_fn(xs, FNS[op], ident, 3)
+            want = oracle.reduce(xs, op, ident, 3)
+            assert got == want, op
+    assert oracle.reduce_fn(x[:0], FNS["plus"], 0, 5) == 5
+
+
+def _affine(a, b):
+    """Maps x -> m*x + c mod 2^32 packed as c << 32 | m; a then b."""
+    m32 = 0xFFFFFFFF
+    am, ac, bm, bc = a & m32, a >> 32, b & m32, b >> 32
+    return (((bm * ac + bc) & m32) << 32) | ((am * bm) & m32)
+
+
+def test_scan_fn_non_commutative_order_closed_form():
+    """Affine-map composition (associative, not commutative): out[i] must be
+    the map 'x[0], then x[1], ..., then x[i]', checked by applying the maps
+    to points one by one; a scan that swapped operands fails."""
+    rng = np.random.default_rng(5)
+    x = rng.integers(0, 2**64, size=500, dtype=np.uint64, endpoint=False)
+    out = oracle.scan_fn(x, _affine, 1)
+    ex = oracle.scan_fn(x, _affine, 1, exclusive=True)
+    assert int(ex[0]) == 1
+    for t in (0, 1, 12345):
+        v = t
+        for i in range(x.size):
+            m, c = int(x[i]) & 0xFFFFFFFF, int(x[i]) >> 32
+            v = (m * v + c) & 0xFFFFFFFF
+            g = int(out[i])
+            assert ((g & 0xFFFFFFFF) * t + (g >> 32)) & 0xFFFFFFFF == v
+    swapped = oracle.scan_fn(x, lambda a, b: _affine(b, a), 1)
+    assert not np.array_equal(swapped, out)
+
+
 # ---- histogram ---------------------------------------------------------------
 
 @pytest.mark.parametrize("n", SIZES)
