@@ -5,7 +5,8 @@ Main line: stable key-value radix sort (clc_sort) of 2^28 uint32 keys
 uniform over 32 bits with uint32 values 0..n-1, ascending.
 
     python bench.py [--steps K] [--warmup W] [--log2n 28] [--key u32|u64]
-                    [--legs u64,learned,segmented,batched | none] [--no-cpu]
+                    [--legs u64,learned,segmented,batched,primitives | none]
+                    [--no-cpu]
 
 A step is one clc_sort call on inputs already resident in HBM. Before each
 step the pristine keys/values are restored by a device-to-device copy that
@@ -23,6 +24,11 @@ workloads of the same path, each timed the same way:
              clamped to [1, 2^20], uint32 keys + segment-local indices:
              one captured clc_graph launch against one clc_sort call per
              segment; outputs compared bit for bit;
+  primitives clc_scan (exclusive, "(a + b)" and the run-time compiled
+             "b + a + 0"), clc_reduce and clc_histogram_even (256 bins) on
+             2^28 uint32: kernel time (library hipEvent profiler), HBM
+             roofline fraction at 8 / 4 / 4 algorithmic bytes per element,
+             outputs checked by difference / int64 sum / bincount;
   batched    clc_learned_segmented_sort against clc_segmented_sort and
              against one clc_sort call per segment on 4096 float32 segments
              with log-uniform sizes 2^10..2^22 (about 2^31 pairs), each from
@@ -505,6 +511,73 @@ def segmented_leg(torch, clc_hip, ctx, steps, warmup):
     return out
 
 
+def primitives_leg(torch, clc_hip, ctx, steps, warmup):
+    """clc_scan / clc_reduce / clc_histogram_even on 2^28 uint32 keys uniform
+    over 32 bits. Kernel times come from the library's per-launch hipEvent
+    profiler (average over `steps` calls after `warmup`)."""
+    n = 1 << 28
+    g = torch.Generator(device="cuda")
+    g.manual_seed(2024)
+    x = torch.randint(-2**31, 2**31 - 1, (n,), dtype=torch.int32, device="cuda",
+                      generator=g)
+    out = torch.empty_like(x)
+    hist = torch.zeros(256, dtype=torch.int32, device="cuda")
+    mask = 0xFFFFFFFF
+
+    def timed(fn, names):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        ctx.reset_timings()
+        ctx.set_profiling(True)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) * 1e3 / steps
+        t = ctx.kernel_timings()
+        ctx.set_profiling(False)
+        ms = {k: t[k]["total_ms"] / t[k]["launches"] for k in names if k in t}
+        return ms, wall
+
+    def rate(ms, bytes_per):
+        gbs = bytes_per * n / (ms * 1e6)
+        return {"kernel_ms": round(ms, 4), "gelem_s": round(n / (ms * 1e6), 2),
+                "achieved_gbs": round(gbs, 1), "roofline_frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+    res = {"n": n, "dtype": "u32"}
+    for label, op, kname in (("scan_exclusive_plus", "(a + b)", "scan"),
+                             ("scan_exclusive_rtc_expr", "b + a + 0", "rtc_scan")):
+        say("leg primitives", label)
+        ms, wall = timed(lambda: ctx.scan(x, out, op, "0", True, type_code=clc_hip.U32),
+                         [kname])
+        d = (out[1:].long() - out[:-1].long()) & mask
+        ok = bool(out[0].item() == 0) and bool(torch.equal(d, x[:-1].long() & mask))
+        res[label] = dict(rate(ms[kname], 8), call_ms=round(wall, 4), output_check=ok)
+    say("leg primitives reduce")
+    want = int(x.long().sum().item()) & mask
+    got = []
+    ms, wall = timed(lambda: got.append(ctx.reduce(x, "(a + b)", "0", 0,
+                                                   type_code=clc_hip.U32)),
+                     ["reduce", "reduce_partials"])
+    res["reduce_plus"] = dict(rate(ms["reduce"] + ms.get("reduce_partials", 0.0), 4),
+                              call_ms_incl_host_sync=round(wall, 4),
+                              output_check=all(int(v) == want for v in got))
+    say("leg primitives histogram_even")
+    ms, wall = timed(lambda: (hist.zero_(), ctx.histogram_even(
+        x, hist, 0.0, 4294967296.0, type_code=clc_hip.U32)), ["histogram_even"])
+    # The reference bins in float: x rounds to float first (values within
+    # 128 of 2^32 round to 2^32 and fall outside [0, 2^32)); scale 2^-24 is
+    # exact.
+    xf = (x.long() & mask).to(torch.float32)
+    inside = xf < 4294967296.0
+    bins = (xf[inside] * (1.0 / 16777216.0)).long()
+    ok = bool(torch.equal(hist.long(), torch.bincount(bins, minlength=256)))
+    res["histogram_even_256"] = dict(rate(ms["histogram_even"], 4), call_ms=round(wall, 4),
+                                     output_check=ok)
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
@@ -514,9 +587,9 @@ def main() -> int:
                     help="log2 of the key count (default 28 for u32, 27 for u64)")
     ap.add_argument("--key", choices=["u32", "u64"], default="u32",
                     help="u32: full 32-bit keys; u64: keys uniform over [0, 2^48)")
-    ap.add_argument("--legs", default="u64,learned,segmented,batched",
+    ap.add_argument("--legs", default="u64,learned,segmented,batched,primitives",
                     help="extra workloads to report: comma list of u64, learned, "
-                         "segmented, or 'none'")
+                         "segmented, batched, primitives, or 'none'")
     ap.add_argument("--keys-only", action="store_true",
                     help="sort keys without values (main line only)")
     ap.add_argument("--learned-log2n", type=int, default=28)
@@ -530,7 +603,7 @@ def main() -> int:
         return 2
     legs = [] if args.legs == "none" else [x for x in args.legs.split(",") if x]
     for x in legs:
-        if x not in ("u64", "learned", "segmented", "batched"):
+        if x not in ("u64", "learned", "segmented", "batched", "primitives"):
             ap.error(f"unknown leg {x}")
 
     import torch
@@ -607,6 +680,11 @@ def main() -> int:
         lb = learned_batched_leg(torch, clc_hip, ctx, 3, 1)
         ok = ok and lb["bit_identical_to_plain"]
         line["legs"]["learned_batched"] = lb
+    if "primitives" in legs:
+        pr = primitives_leg(torch, clc_hip, ctx, max(5, args.steps // 2), 2)
+        ok = ok and all(v["output_check"] for v in pr.values() if isinstance(v, dict))
+        line["legs"]["primitives"] = pr
+        torch.cuda.empty_cache()
     line["extra"]["output_check"] = ok
     if not args.no_cpu:
         say("cpu baseline")

@@ -522,8 +522,10 @@ clc_status clc_operator_supported(clc_type type, const char* op,
         clc::op_code code;
         unsigned char id[8];
         if (!canonical(type, op, identity, code, id))
-            (void)clc::rtc::compile(clc::rtc::operator_source(type, op, identity, false),
-                                    "gfx950");
+            (void)clc::rtc::compile(
+                clc::rtc::operator_source(type, op, identity,
+                                          clc::rtc::WITH_SCAN | clc::rtc::WITH_REDUCE),
+                "gfx950");
     });
 }
 

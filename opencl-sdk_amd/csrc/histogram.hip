@@ -94,6 +94,7 @@ void histogram_typed(context& ctx, const void* in, size_t n, uint32_t* hist,
                             size_t(ctx.facts().cu_count) * 4)));
     const int vec_ok = reinterpret_cast<uintptr_t>(in) % 16 == 0 ? 1 : 0;
     const T* src = static_cast<const T*>(in);
+    timed_launch tl(ctx, "histogram_even");
     if (local)
         hipLaunchKernelGGL((histogram_kernel<T, ACC, true, BLOCK>), dim3(groups),
                            dim3(BLOCK), lds, ctx.stream(), src, n, hist, bins,

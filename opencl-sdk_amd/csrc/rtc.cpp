@@ -317,7 +317,7 @@ std::string base_arch(const std::string& arch)
 }
 
 std::string operator_source(int type, const std::string& op,
-                            const std::string& identity, bool scan)
+                            const std::string& identity, int kernels)
 {
     std::string s;
     s += "#include <cstdint>\n#include <type_traits>\n";
@@ -337,20 +337,22 @@ std::string operator_source(int type, const std::string& op,
          "#line 1 \"identity\"\n";
     s += identity;
     s += "\n#line 1 \"clc_rtc_kernels\"\n);\n}\n} // namespace clc_ocl\n";
-    s += "static_assert(clc::dev::SCAN_BLOCK == 256, \"launch bounds\");\n";
-    if (scan)
-        s += "extern \"C\" __global__ __launch_bounds__(256) void clc_rtc_scan(\n"
+    if (kernels & WITH_SCAN)
+        s += "extern \"C\" __global__\n"
+             "__launch_bounds__(clc::dev::SCAN_BLOCK) void clc_rtc_scan(\n"
              "    const clc_ocl::T* in, clc_ocl::T* out, size_t n, int exclusive,\n"
              "    int vec_ok, uint64_t* status, uint32_t* tile_counter)\n{\n"
-             "    clc::dev::scan_tile<clc_ocl::T, clc_ocl::user_op, 256,\n"
+             "    clc::dev::scan_tile<clc_ocl::T, clc_ocl::user_op, clc::dev::SCAN_BLOCK,\n"
              "                        clc::dev::scan_items<clc_ocl::T>()>(\n"
              "        in, out, n, clc_ocl::user_identity(), exclusive, vec_ok,\n"
              "        status, tile_counter);\n}\n";
-    else
-        s += "extern \"C\" __global__ __launch_bounds__(256) void clc_rtc_reduce(\n"
+    if (kernels & WITH_REDUCE)
+        s += "extern \"C\" __global__\n"
+             "__launch_bounds__(clc::dev::REDUCE_BLOCK) void clc_rtc_reduce(\n"
              "    const clc_ocl::T* in, size_t n, int vec_ok, clc_ocl::T* out,\n"
              "    int apply_init, clc_ocl::T init)\n{\n"
-             "    clc::dev::reduce_partial<clc_ocl::T, clc_ocl::user_op, 256>(\n"
+             "    clc::dev::reduce_partial<clc_ocl::T, clc_ocl::user_op,\n"
+             "                             clc::dev::REDUCE_BLOCK>(\n"
              "        in, n, clc_ocl::user_identity(), vec_ok, out, apply_init, init);\n}\n";
     return s;
 }

@@ -36,15 +36,23 @@ namespace rtc {
 constexpr const char* SCAN_KERNEL = "clc_rtc_scan";
 constexpr const char* REDUCE_KERNEL = "clc_rtc_reduce";
 
-//! Source of the scan (scan = true) or reduce kernel over elements of
-//! type (a type_code) with the operator expression op over a and b and the
-//! identity expression identity, both in OpenCL C scalar syntax:
+//! Kernels a generated source holds (bit set).
+enum kernel_set : int
+{
+    WITH_SCAN = 1,
+    WITH_REDUCE = 2
+};
+
+//! Source of the scan and/or reduce kernel (kernels: a kernel_set) over
+//! elements of type (a type_code) with the operator expression op over a
+//! and b and the identity expression identity, both in OpenCL C scalar
+//! syntax:
 //!   scan:   clc_rtc_scan(const T* in, T* out, size_t n, int exclusive,
 //!                        int vec_ok, uint64_t* status, uint32_t* counter)
 //!   reduce: clc_rtc_reduce(const T* in, size_t n, int vec_ok, T* out,
 //!                          int apply_init, T init)
 std::string operator_source(int type, const std::string& op,
-                            const std::string& identity, bool scan);
+                            const std::string& identity, int kernels);
 
 //! Compiles source with hipRTC for arch (a gfx target name such as
 //! gfx950); returns the code object. Throws build_error with the log.

@@ -5,11 +5,13 @@
 // associative operator including non-commutative ones (operands always
 // combine in sequence order), exclusive output seeded with the identity.
 // The reference makes three launches per level and recurses over tile
-// totals (two full read+write sweeps). Here one launch does it: each tile
-// loads 16 bytes per lane per load, reduces its items, publishes its
-// aggregate, looks back over earlier tiles' published aggregates/prefixes
-// and writes its outputs once (decoupled look-back; 2 x 4 bytes/element of
-// HBM traffic for u32).
+// totals (two full read+write sweeps). Here one launch does it: each
+// 512-thread tile reads its 64 KiB (u32) with coalesced 16-byte loads,
+// reduces them, publishes its aggregate, looks back over 16 earlier tiles'
+// published aggregates/prefixes at a time and writes its outputs once
+// (decoupled look-back; 2 x 4 bytes/element of HBM traffic for u32). The
+// kernel bodies are in scan_kernels.hpp, shared with the run-time compiled
+// operator expressions.
 //
 // reduce: the reference's contract (detail::reduce, Algorithm.cpp:400-442):
 // op(init, fold) with an order left unspecified (the operator must be
@@ -91,6 +93,7 @@ void scan_typed(context& ctx, const void* in, void* out, size_t n,
     temp_buffer tmp(ctx.resource(), bytes);
     char* t = tmp.as<char>();
     detail::zero_fill(ctx.stream(), t, bytes);
+    timed_launch tl(ctx, "scan");
     hipLaunchKernelGGL((scan_kernel<T, OP, BLOCK, ITEMS>), dim3(tiles), dim3(BLOCK),
                        0, ctx.stream(), static_cast<const T*>(in),
                        static_cast<T*>(out), n, id, exclusive ? 1 : 0, vec_ok,
@@ -103,23 +106,29 @@ template <typename T, int OP>
 void reduce_typed(context& ctx, const void* in, size_t n, const void* identity,
                   const void* init, void* result)
 {
-    constexpr int BLOCK = SCAN_BLOCK;
+    constexpr int BLOCK = REDUCE_BLOCK;
     T id, ini;
     std::memcpy(&id, identity, sizeof(T));
     std::memcpy(&ini, init, sizeof(T));
     const size_t per_block = size_t(BLOCK) * 16;
     const uint32_t groups = static_cast<uint32_t>(std::max<size_t>(
         1, std::min<size_t>((n + per_block - 1) / per_block,
-                            size_t(ctx.facts().cu_count) * 4)));
+                            size_t(ctx.facts().cu_count) * 8)));
     temp_buffer partials(ctx.resource(), (groups + 1) * sizeof(T));
     T* part = partials.as<T>();
     const int vec_ok = reinterpret_cast<uintptr_t>(in) % 16 == 0 ? 1 : 0;
-    hipLaunchKernelGGL((reduce_kernel<T, OP, BLOCK>), dim3(groups), dim3(BLOCK), 0,
-                       ctx.stream(), static_cast<const T*>(in), n, id, vec_ok,
-                       part, 0, ini);
-    hipLaunchKernelGGL((reduce_kernel<T, OP, BLOCK>), dim3(1), dim3(BLOCK), 0,
-                       ctx.stream(), part, size_t(groups), id, 0, part + groups,
-                       1, ini);
+    {
+        timed_launch tl(ctx, "reduce");
+        hipLaunchKernelGGL((reduce_kernel<T, OP, BLOCK>), dim3(groups), dim3(BLOCK), 0,
+                           ctx.stream(), static_cast<const T*>(in), n, id, vec_ok,
+                           part, 0, ini);
+    }
+    {
+        timed_launch tl(ctx, "reduce_partials");
+        hipLaunchKernelGGL((reduce_kernel<T, OP, BLOCK>), dim3(1), dim3(BLOCK), 0,
+                           ctx.stream(), part, size_t(groups), id, 0, part + groups,
+                           1, ini);
+    }
     CLC_HIP(hipGetLastError());
     CLC_HIP(hipMemcpyAsync(ctx.pinned(), part + groups, sizeof(T),
                            hipMemcpyDeviceToHost, ctx.stream()));
@@ -238,11 +247,10 @@ void scan_expr(context& ctx, const void* in, void* out, std::size_t n, int type,
         throw std::invalid_argument("clc_scan: null buffer");
     ctx.activate();
     hipFunction_t f = ctx.programs().function(
-        rtc::operator_source(type, op, identity, true), rtc::SCAN_KERNEL);
-    // Same tiling as the ahead-of-time scan (scan_items<T>() elements a
-    // thread, SCAN_BLOCK threads a tile).
-    const std::size_t items = es >= 4 ? 64 / es : 16;
-    const std::size_t tile = std::size_t(SCAN_BLOCK) * items;
+        rtc::operator_source(type, op, identity, rtc::WITH_SCAN), rtc::SCAN_KERNEL);
+    // Same tiling as the ahead-of-time scan.
+    const std::size_t tile =
+        std::size_t(SCAN_BLOCK) * scan_items_of(static_cast<unsigned>(es));
     const std::size_t tiles = (n + tile - 1) / tile;
     int vec_ok = (reinterpret_cast<uintptr_t>(in) % 16 == 0
                   && reinterpret_cast<uintptr_t>(out) % 16 == 0)
@@ -276,11 +284,11 @@ void reduce_expr(context& ctx, const void* in, std::size_t n, int type,
     if (in == nullptr) throw std::invalid_argument("clc_reduce: null buffer");
     ctx.activate();
     hipFunction_t f = ctx.programs().function(
-        rtc::operator_source(type, op, identity, false), rtc::REDUCE_KERNEL);
-    const std::size_t per_block = std::size_t(SCAN_BLOCK) * 16;
+        rtc::operator_source(type, op, identity, rtc::WITH_REDUCE), rtc::REDUCE_KERNEL);
+    const std::size_t per_block = std::size_t(REDUCE_BLOCK) * 16;
     const uint32_t groups = static_cast<uint32_t>(std::max<std::size_t>(
         1, std::min<std::size_t>((n + per_block - 1) / per_block,
-                                 std::size_t(ctx.facts().cu_count) * 4)));
+                                 std::size_t(ctx.facts().cu_count) * 8)));
     temp_buffer partials(ctx.resource(), (groups + 1) * es);
     char* part = partials.as<char>();
     unsigned char ini[8] = {};
@@ -291,7 +299,7 @@ void reduce_expr(context& ctx, const void* in, std::size_t n, int type,
         void* out = part;
         void* args[] = { &in, &n, &vec_ok, &out, &apply, ini };
         timed_launch tl(ctx, "rtc_reduce");
-        CLC_HIP(hipModuleLaunchKernel(f, groups, 1, 1, SCAN_BLOCK, 1, 1, 0,
+        CLC_HIP(hipModuleLaunchKernel(f, groups, 1, 1, REDUCE_BLOCK, 1, 1, 0,
                                       ctx.stream(), args, nullptr));
     }
     {
@@ -300,8 +308,8 @@ void reduce_expr(context& ctx, const void* in, std::size_t n, int type,
         int vec_ok = 0, apply = 1;
         void* out = part + std::size_t(groups) * es;
         void* args[] = { &src, &count, &vec_ok, &out, &apply, ini };
-        timed_launch tl(ctx, "rtc_reduce");
-        CLC_HIP(hipModuleLaunchKernel(f, 1, 1, 1, SCAN_BLOCK, 1, 1, 0, ctx.stream(),
+        timed_launch tl(ctx, "rtc_reduce_partials");
+        CLC_HIP(hipModuleLaunchKernel(f, 1, 1, 1, REDUCE_BLOCK, 1, 1, 0, ctx.stream(),
                                       args, nullptr));
     }
     CLC_HIP(hipMemcpyAsync(ctx.pinned(), part + std::size_t(groups) * es, es,

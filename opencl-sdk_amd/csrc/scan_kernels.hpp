@@ -15,15 +15,17 @@
 namespace clc {
 namespace dev {
 
-//! Threads per scan / reduce workgroup.
-constexpr int SCAN_BLOCK = 256;
+//! Threads per scan workgroup and per reduce workgroup.
+constexpr int SCAN_BLOCK = 512;
+constexpr int REDUCE_BLOCK = 256;
 
-//! Elements per thread of a scan tile: 64 bytes per thread for 4- and
-//! 8-byte elements (four 16-byte loads), 16 elements for narrower ones.
-template <typename T> constexpr int scan_items()
-{
-    return sizeof(T) >= 4 ? int(64 / sizeof(T)) : 16;
-}
+//! Elements per thread of a scan tile for elements of `bytes` bytes: 128
+//! bytes a thread (eight 16-byte loads) for 4- and 8-byte elements, 32
+//! elements for narrower ones. Tiles of 16K+ elements (64 KiB of u32) keep
+//! the look-back's latency per tile small against the tile's traffic: on
+//! 2^28 u32, 4K-element tiles ran at half the speed of 16K-element ones.
+constexpr int scan_items_of(unsigned bytes) { return bytes >= 4 ? int(128 / bytes) : 32; }
+template <typename T> constexpr int scan_items() { return scan_items_of(sizeof(T)); }
 
 template <typename T> __device__ __forceinline__ T shfl_up_elem(T x, int o)
 {
@@ -87,21 +89,109 @@ template <typename T, typename Op> __device__ __forceinline__ T wave_scan_op(T x
 }
 
 // ---- scan status words -------------------------------------------------------
-// A tile's published value is up to 8 bytes; it is split into two 64-bit
-// words, each carrying the flag in its top half, so each word is read and
-// written atomically and a reader accepts the pair only when both carry the
-// same flag (flags only move 0 -> AGG -> PREFIX).
+// Each tile publishes its aggregate, then its inclusive prefix, with a flag
+// in the top half of 64-bit words read and written atomically. Elements of
+// up to 4 bytes share one word with the flag; 8-byte elements are split
+// over two words, each carrying the flag, and a reader accepts the pair only
+// when both carry the same flag (flags only move 0 -> AGG -> PREFIX).
 
 constexpr uint64_t SCAN_FLAG_AGG = uint64_t(1) << 32;
 constexpr uint64_t SCAN_FLAG_PREFIX = uint64_t(2) << 32;
+constexpr uint64_t SCAN_LOW = 0xFFFFFFFFu;
 
 template <typename T>
 __device__ __forceinline__ void publish_status(uint64_t* st, T v, uint64_t flag)
 {
     uint64_t bits = 0;
     __builtin_memcpy(&bits, &v, sizeof(T));
-    store_agent(st + 1, flag | (bits >> 32));
-    store_agent(st, flag | (bits & uint64_t(0xFFFFFFFFu)));
+    if constexpr (sizeof(T) > 4) store_agent(st + 1, flag | (bits >> 32));
+    store_agent(st, flag | (bits & SCAN_LOW));
+}
+
+//! Reads tile p's status: its flag (0 while not ready) and value bits.
+template <typename T>
+__device__ __forceinline__ uint64_t read_status(const uint64_t* status, long long p,
+                                                uint64_t& bits)
+{
+    const uint64_t lo = load_agent(status + 2 * p);
+    uint64_t f = lo & ~SCAN_LOW;
+    bits = lo & SCAN_LOW;
+    if constexpr (sizeof(T) > 4)
+    {
+        const uint64_t hi = load_agent(status + 2 * p + 1);
+        if ((hi & ~SCAN_LOW) != f) f = 0; // torn pair: not ready yet
+        bits |= hi << 32;
+    }
+    return f;
+}
+
+//! Combination, earliest first, of the values published by the tiles
+//! before tile (tile > 0), by one wave: each round lanes 0..WIN-1 read the
+//! status words of WIN consecutive predecessors at once (lane l: tile - 1 -
+//! l - WIN*r). A round ends the walk when some lane holds a prefix and every
+//! lane before it an aggregate; when all WIN hold aggregates they are folded
+//! in and the walk goes on; otherwise the round reads the same predecessors
+//! again. Lanes are combined by an ordered tree (a higher lane is an earlier
+//! tile, so it is the left operand). The result is valid in lane 0.
+template <typename T, typename Op, int WIN>
+__device__ __forceinline__ T window_lookback(const uint64_t* status, size_t tile)
+{
+    static_assert(WIN >= 1 && WIN <= WAVE && (WIN & (WIN - 1)) == 0, "window");
+    const uint32_t lane = lane_id();
+    const bool reader = lane < static_cast<uint32_t>(WIN);
+    T acc{};
+    bool got = false;
+    long long base = static_cast<long long>(tile) - 1;
+    for (;;)
+    {
+        const long long p = base - static_cast<long long>(lane);
+        uint64_t bits = 0, f = SCAN_FLAG_PREFIX; // before tile 0: ends the walk
+        if (reader && p >= 0) f = read_status<T>(status, p, bits);
+        const uint64_t prefix = __ballot(reader && f == SCAN_FLAG_PREFIX);
+        const uint64_t waiting = __ballot(reader && f == 0);
+        // Lanes 0..last are needed: up to the first prefix, or all WIN.
+        const int last = prefix ? __builtin_ctzll(prefix) : WIN - 1;
+        const uint64_t needed = (uint64_t(2) << last) - 1; // last < 64
+        if (waiting & needed) continue;
+        T x;
+        __builtin_memcpy(&x, &bits, sizeof(T));
+#pragma unroll
+        for (int o = 1; o < WIN; o <<= 1)
+        {
+            const T y = shfl_down_elem(x, o);
+            if ((lane & (2 * o - 1)) == 0 && static_cast<int>(lane) + o <= last)
+                x = Op::apply(y, x);
+        }
+        acc = got ? Op::apply(x, acc) : x;
+        got = true;
+        if (prefix) return acc;
+        base -= WIN;
+    }
+}
+
+//! Value of lane src of the wave, in every lane.
+template <typename T> __device__ __forceinline__ T lane_value(T x, int src)
+{
+    if constexpr (sizeof(T) <= 4)
+    {
+        uint32_t u = 0;
+        __builtin_memcpy(&u, &x, sizeof(T));
+        u = __builtin_amdgcn_readlane(u, src);
+        T r;
+        __builtin_memcpy(&r, &u, sizeof(T));
+        return r;
+    }
+    else
+    {
+        uint64_t u;
+        __builtin_memcpy(&u, &x, 8);
+        const uint32_t lo = __builtin_amdgcn_readlane(static_cast<uint32_t>(u), src);
+        const uint32_t hi = __builtin_amdgcn_readlane(static_cast<uint32_t>(u >> 32), src);
+        u = (static_cast<uint64_t>(hi) << 32) | lo;
+        T r;
+        __builtin_memcpy(&r, &u, 8);
+        return r;
+    }
 }
 
 //! One tile of a single-pass scan (decoupled look-back): the body of a
@@ -109,15 +199,24 @@ __device__ __forceinline__ void publish_status(uint64_t* st, T v, uint64_t flag)
 //! elements. status holds 2 zeroed words per tile, tile_counter one zeroed
 //! word (tiles are numbered in the order workgroups start, so every tile a
 //! workgroup waits on belongs to a workgroup already running).
-template <typename T, typename Op, int BLOCK, int ITEMS>
+//!
+//! Layout: wave w owns the w-th span of ITEMS*64 consecutive elements and
+//! reads it with J = ITEMS/VEC fully coalesced 16-byte loads (load j, lane l:
+//! elements j*64*VEC + l*VEC .. +VEC-1 of the span). Each load's VEC
+//! elements are folded in the thread, the folds scanned across the wave,
+//! the J span pieces chained in order, then the W spans in order, then the
+//! tile prefix from the look-back is put in front.
+template <typename T, typename Op, int BLOCK, int ITEMS, int WIN = 16>
 __device__ __forceinline__ void scan_tile(
     const T* in, T* out, size_t n, T identity, int exclusive, int vec_ok,
     uint64_t* status, uint32_t* tile_counter)
 {
     constexpr int W = BLOCK / WAVE;
-    constexpr size_t TILE = size_t(BLOCK) * ITEMS;
     constexpr int VEC = 16 / sizeof(T); // elements per 16-byte load
     static_assert(ITEMS % VEC == 0, "items per thread");
+    constexpr int J = ITEMS / VEC;
+    constexpr size_t SPAN = size_t(WAVE) * ITEMS;
+    constexpr size_t TILE = size_t(BLOCK) * ITEMS;
     __shared__ T s_wave[W];
     __shared__ T s_prefix;
     __shared__ uint32_t s_tile;
@@ -126,7 +225,7 @@ __device__ __forceinline__ void scan_tile(
     if (tid == 0) s_tile = fetch_add_agent(tile_counter, 1u);
     __syncthreads();
     const size_t tile = s_tile;
-    const size_t first = tile * TILE + size_t(tid) * ITEMS;
+    const size_t first = tile * TILE + wave * SPAN + size_t(lane) * VEC;
     const bool full = (tile + 1) * TILE <= n && vec_ok;
 
     T x[ITEMS];
@@ -134,29 +233,40 @@ __device__ __forceinline__ void scan_tile(
     {
         const uint4* src = reinterpret_cast<const uint4*>(in + first);
 #pragma unroll
-        for (int j = 0; j < ITEMS / VEC; ++j)
+        for (int j = 0; j < J; ++j)
         {
-            const uint4 q = src[j];
+            const uint4 q = src[j * WAVE];
             __builtin_memcpy(&x[j * VEC], &q, 16);
         }
     }
     else
     {
 #pragma unroll
-        for (int j = 0; j < ITEMS; ++j)
-            x[j] = first + j < n ? in[first + j] : identity;
+        for (int j = 0; j < J; ++j)
+#pragma unroll
+            for (int e = 0; e < VEC; ++e)
+            {
+                const size_t i = first + size_t(j) * WAVE * VEC + e;
+                x[j * VEC + e] = i < n ? in[i] : identity;
+            }
     }
 
     // Prefixes are computed unseeded (no identity inside), so operators
     // without a two-sided identity (the projections) still give the
     // sequential result; the identity is applied once, on output.
-    // Thread fold, then wave and block exclusive scans of the folds.
-    T agg = x[0];
+    T inc[J];  // inclusive scan of load j's folds across the wave
+    T span{};  // the wave's span aggregate (wave-uniform)
 #pragma unroll
-    for (int j = 1; j < ITEMS; ++j) agg = Op::apply(agg, x[j]);
-    const T inc = wave_scan_op<T, Op>(agg);
-    if (lane == WAVE - 1) s_wave[wave] = inc;
-    const T excl_lane = shfl_up_elem(inc, 1); // meaningful for lane > 0
+    for (int j = 0; j < J; ++j)
+    {
+        T f = x[j * VEC];
+#pragma unroll
+        for (int e = 1; e < VEC; ++e) f = Op::apply(f, x[j * VEC + e]);
+        inc[j] = wave_scan_op<T, Op>(f);
+        const T total = lane_value(inc[j], WAVE - 1);
+        span = j == 0 ? total : Op::apply(span, total);
+    }
+    if (lane == 0) s_wave[wave] = span;
     __syncthreads();
     T wave_prefix = s_wave[0]; // meaningful for wave > 0
     for (int w = 1; w < W; ++w)
@@ -164,82 +274,88 @@ __device__ __forceinline__ void scan_tile(
     T block_agg = s_wave[0];
     for (int w = 1; w < W; ++w) block_agg = Op::apply(block_agg, s_wave[w]);
 
-    // Look-back by thread 0.
-    if (tid == 0)
+    if (wave == 0)
     {
         uint64_t* mine = status + 2 * tile;
         if (tile == 0)
         {
-            publish_status(mine, block_agg, SCAN_FLAG_PREFIX);
-            s_prefix = identity;
+            if (lane == 0)
+            {
+                publish_status(mine, block_agg, SCAN_FLAG_PREFIX);
+                s_prefix = identity;
+            }
         }
         else
         {
-            publish_status(mine, block_agg, SCAN_FLAG_AGG);
-            T acc = identity;
-            bool got = false;
-            size_t p = tile - 1;
-            for (;;)
+            if (lane == 0) publish_status(mine, block_agg, SCAN_FLAG_AGG);
+            const T acc = window_lookback<T, Op, WIN>(status, tile);
+            if (lane == 0)
             {
-                const uint64_t lo = load_agent(status + 2 * p);
-                const uint64_t hi = load_agent(status + 2 * p + 1);
-                const uint64_t f = lo & ~uint64_t(0xFFFFFFFFu);
-                if (f == 0 || f != (hi & ~uint64_t(0xFFFFFFFFu))) continue;
-                const uint64_t bits = (lo & uint64_t(0xFFFFFFFFu)) | (hi << 32);
-                T val;
-                __builtin_memcpy(&val, &bits, sizeof(T));
-                acc = got ? Op::apply(val, acc) : val;
-                got = true;
-                if (f == SCAN_FLAG_PREFIX) break;
-                --p;
+                publish_status(mine, Op::apply(acc, block_agg), SCAN_FLAG_PREFIX);
+                s_prefix = acc;
             }
-            publish_status(mine, Op::apply(acc, block_agg), SCAN_FLAG_PREFIX);
-            s_prefix = acc;
         }
     }
     __syncthreads();
 
-    // pre = tile prefix op wave prefix op lane prefix, each when present.
-    T pre = s_prefix;
+    // run = tile prefix op earlier spans op earlier pieces of this span,
+    // each when present (wave-uniform).
+    T run = s_prefix;
     bool has = tile > 0;
     if (wave > 0)
     {
-        pre = has ? Op::apply(pre, wave_prefix) : wave_prefix;
-        has = true;
-    }
-    if (lane > 0)
-    {
-        pre = has ? Op::apply(pre, excl_lane) : excl_lane;
+        run = has ? Op::apply(run, wave_prefix) : wave_prefix;
         has = true;
     }
     // out = identity op (in[0] op ... op in[i]) (inclusive) or
     //       identity op (in[0] op ... op in[i-1]) (exclusive; identity at 0).
     T y[ITEMS];
 #pragma unroll
-    for (int j = 0; j < ITEMS; ++j)
+    for (int j = 0; j < J; ++j)
     {
-        const T next = has ? Op::apply(pre, x[j]) : x[j];
-        y[j] = exclusive ? (has ? Op::apply(identity, pre) : identity)
-                         : Op::apply(identity, next);
-        pre = next;
+        const T excl = shfl_up_elem(inc[j], 1); // lanes below, lane > 0
+        T pre = run;
+        bool hp = has;
+        if (lane > 0)
+        {
+            pre = hp ? Op::apply(pre, excl) : excl;
+            hp = true;
+        }
+#pragma unroll
+        for (int e = 0; e < VEC; ++e)
+        {
+            const T v = x[j * VEC + e];
+            const T next = hp ? Op::apply(pre, v) : v;
+            y[j * VEC + e] = exclusive ? (hp ? Op::apply(identity, pre) : identity)
+                                       : Op::apply(identity, next);
+            pre = next;
+            hp = true;
+        }
+        const T total = lane_value(inc[j], WAVE - 1);
+        run = has ? Op::apply(run, total) : total;
         has = true;
     }
     if (full)
     {
         uint4* dst = reinterpret_cast<uint4*>(out + first);
 #pragma unroll
-        for (int j = 0; j < ITEMS / VEC; ++j)
+        for (int j = 0; j < J; ++j)
         {
             uint4 q;
             __builtin_memcpy(&q, &y[j * VEC], 16);
-            dst[j] = q;
+            dst[j * WAVE] = q;
         }
     }
     else
     {
 #pragma unroll
-        for (int j = 0; j < ITEMS; ++j)
-            if (first + j < n) out[first + j] = y[j];
+        for (int j = 0; j < J; ++j)
+#pragma unroll
+            for (int e = 0; e < VEC; ++e)
+            {
+                const size_t i = first + size_t(j) * WAVE * VEC + e;
+                if (i < n) out[i] = y[j * VEC + e];
+            }
     }
 }
 
@@ -277,9 +393,13 @@ __device__ __forceinline__ void reduce_partial(
     size_t start = 0;
     if (vec_ok)
     {
+        // One 16-byte load a thread per grid stride: on 2^28 u32 this ran at
+        // 6.4 TB/s with 8 workgroups per CU; four loads in flight a thread
+        // (grid-strided or block-contiguous) ran at 5.2-6.2 TB/s.
         const size_t nvec = n / VEC;
         const uint4* v = reinterpret_cast<const uint4*>(in);
-        for (size_t i = size_t(blockIdx.x) * BLOCK + threadIdx.x; i < nvec; i += stride)
+        size_t i = size_t(blockIdx.x) * BLOCK + threadIdx.x;
+        for (; i < nvec; i += stride)
         {
             const uint4 q = v[i];
             T e[VEC];
