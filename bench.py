@@ -25,10 +25,12 @@ workloads of the same path, each timed the same way:
              one captured clc_graph launch against one clc_sort call per
              segment; outputs compared bit for bit;
   primitives clc_scan (exclusive, "(a + b)" and the run-time compiled
-             "b + a + 0"), clc_reduce and clc_histogram_even (256 bins) on
-             2^28 uint32: kernel time (library hipEvent profiler), HBM
-             roofline fraction at 8 / 4 / 4 algorithmic bytes per element,
-             outputs checked by difference / int64 sum / bincount;
+             "b + a + 0"), clc_reduce, clc_copy_if ("(x & 1) == 0") and
+             clc_histogram_even (256 bins) on 2^28 uint32: kernel time
+             (library hipEvent profiler), HBM roofline fraction at 8 / 4 /
+             4 + 4 x kept fraction / 4 algorithmic bytes per element,
+             outputs checked by difference / int64 sum / torch mask
+             selection / bincount;
   batched    clc_learned_segmented_sort against clc_segmented_sort and
              against one clc_sort call per segment on 4096 float32 segments
              with log-uniform sizes 2^10..2^22 (about 2^31 pairs), each from
@@ -563,6 +565,19 @@ def primitives_leg(torch, clc_hip, ctx, steps, warmup):
     res["reduce_plus"] = dict(rate(ms["reduce"] + ms.get("reduce_partials", 0.0), 4),
                               call_ms_incl_host_sync=round(wall, 4),
                               output_check=all(int(v) == want for v in got))
+    say("leg primitives copy_if")
+    counts = []
+    ms, wall = timed(lambda: counts.append(ctx.copy_if(x, out, "(x & 1) == 0",
+                                                       type_code=clc_hip.U32)),
+                     ["rtc_copy_if"])
+    want = x[(x & 1) == 0]
+    kept = want.numel()
+    ok = all(c == kept for c in counts) and bool(torch.equal(out[:kept], want))
+    del want
+    # Algorithmic bytes: read every element, write the kept ones.
+    res["copy_if_even"] = dict(rate(ms["rtc_copy_if"], 4 + 4 * kept / n),
+                               kept=kept, call_ms_incl_host_sync=round(wall, 4),
+                               output_check=ok)
     say("leg primitives histogram_even")
     ms, wall = timed(lambda: (hist.zero_(), ctx.histogram_even(
         x, hist, 0.0, 4294967296.0, type_code=clc_hip.U32)), ["histogram_even"])

@@ -193,6 +193,17 @@ clc_status clc_scan(clc_context ctx, const void* in, void* out, size_t n,
                     clc_type type, const char* op, const char* identity,
                     int exclusive);
 
+/* Copies the elements x of in[0, n) for which predicate (an OpenCL C
+ * expression of x, as the reference's) is true to the front of out, in
+ * order, and stores how many in *count; blocks until the count is on the
+ * host. out holds n elements and may alias in. The predicate is compiled
+ * once per context (CLC_ERROR_BUILD with the log when it does not compile).
+ * One pass over in (the reference: flags, scan, scatter).
+ * (reference clc_copy_if, compute_c.h:169-171, CApi.cpp:533-545,
+ * Algorithm.cpp:529-567) */
+clc_status clc_copy_if(clc_context ctx, const void* in, void* out, size_t n,
+                       clc_type type, const char* predicate, size_t* count);
+
 /* Stable in-place radix sort of n keys (I32, U32, F32, and as an extension
  * I64, U64, F64) by the reference's order (Algorithm.cpp:459-469: floats by
  * their sign-flipped bits, so -0.0 sorts before +0.0); values may be NULL or
@@ -340,6 +351,11 @@ clc_status clc_learned_segmented_sort(clc_context ctx, void* keys, clc_type key_
  * or CLC_ERROR_INVALID_ARGUMENT. Needs no device. */
 clc_status clc_operator_supported(clc_type type, const char* op,
                                   const char* identity);
+
+/* Whether clc_copy_if accepts predicate for type: CLC_SUCCESS when it
+ * compiles for gfx950, CLC_ERROR_BUILD with the log otherwise, or
+ * CLC_ERROR_INVALID_ARGUMENT. Needs no device. */
+clc_status clc_predicate_supported(clc_type type, const char* predicate);
 
 #ifdef __cplusplus
 }

@@ -90,6 +90,8 @@ SIGNATURES = {
     "clc_context_memory_stats": (_i, [_vp, ctypes.POINTER(MemoryStats)]),
     "clc_context_release_cache": (_i, [_vp]),
     "clc_context_program_stats": (_i, [_vp, ctypes.POINTER(ProgramStats)]),
+    "clc_copy_if": (_i, [_vp, _vp, _vp, _sz, _i, _cp, ctypes.POINTER(_sz)]),
+    "clc_predicate_supported": (_i, [_i, _cp]),
     "clc_context_set_profiling": (_i, [_vp, _i]),
     "clc_context_kernel_timings": (_i, [_vp, ctypes.POINTER(KernelTiming), _sz,
                                         ctypes.POINTER(_sz)]),
@@ -332,6 +334,17 @@ class Context:
                                ini.ctypes.data_as(ctypes.c_void_p),
                                res.ctypes.data_as(ctypes.c_void_p)))
         return res[0]
+
+    def copy_if(self, inp, out, predicate, type_code=None, n=None) -> int:
+        """clc_copy_if: elements of inp satisfying predicate (an expression
+        of x) to the front of out, in order; returns how many."""
+        t = dtype_code(inp.dtype) if type_code is None else type_code
+        n = inp.numel() if n is None else n
+        count = ctypes.c_size_t(0)
+        check(lib().clc_copy_if(self.handle, ctypes.c_void_p(inp.data_ptr()),
+                                ctypes.c_void_p(out.data_ptr()), n, t, _enc(predicate),
+                                ctypes.byref(count)))
+        return count.value
 
     def histogram_even(self, inp, hist, lower, upper, type_code=None, n=None):
         t = dtype_code(inp.dtype) if type_code is None else type_code

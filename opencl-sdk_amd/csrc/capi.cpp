@@ -378,6 +378,25 @@ clc_status clc_scan(clc_context ctx, const void* in, void* out, size_t n,
     });
 }
 
+clc_status clc_copy_if(clc_context ctx, const void* in, void* out, size_t n,
+                       clc_type type, const char* predicate, size_t* count)
+{
+    return guarded([&] {
+        require(ctx != nullptr && predicate != nullptr && count != nullptr,
+                "clc: null argument");
+        *count = clc::detail::copy_if(get(ctx), in, out, n, type, predicate);
+    });
+}
+
+clc_status clc_predicate_supported(clc_type type, const char* predicate)
+{
+    return guarded([&] {
+        require(predicate != nullptr, "clc: null argument");
+        clc::type_size(type); // rejects unknown types
+        (void)clc::rtc::compile(clc::rtc::predicate_source(type, predicate), "gfx950");
+    });
+}
+
 clc_status clc_sort(clc_context ctx, void* keys, clc_type key_type,
                     void* values, size_t n, int descending)
 {

@@ -222,6 +222,11 @@ void scan_expr(context& ctx, const void* in, void* out, std::size_t n, int type,
 void reduce_expr(context& ctx, const void* in, std::size_t n, int type,
                  const char* op, const char* identity, const void* init,
                  void* result);
+//! Copies the elements x of in[0, n) with (predicate) true, an OpenCL C
+//! expression of x, to the front of out in order; returns how many (blocks
+//! until the count is on the host). out may alias in.
+std::size_t copy_if(context& ctx, const void* in, void* out, std::size_t n, int type,
+                    const char* predicate);
 void histogram_even(context& ctx, const void* in, std::size_t n, int type,
                     uint32_t* histogram, std::size_t num_bins, double lower,
                     double upper);

@@ -316,8 +316,11 @@ std::string base_arch(const std::string& arch)
     return colon == std::string::npos ? arch : arch.substr(0, colon);
 }
 
-std::string operator_source(int type, const std::string& op,
-                            const std::string& identity, int kernels)
+namespace {
+
+// Headers, the builtins prelude and "typedef <element type> T;" opening
+// namespace clc_ocl.
+std::string source_head(int type)
 {
     std::string s;
     s += "#include <cstdint>\n#include <type_traits>\n";
@@ -325,6 +328,37 @@ std::string operator_source(int type, const std::string& op,
     s += PRELUDE;
     s += "namespace clc_ocl {\n";
     s += std::string("typedef ") + type_name(type) + " T;\n";
+    return s;
+}
+
+} // namespace
+
+std::string predicate_source(int type, const std::string& predicate)
+{
+    std::string s = source_head(type);
+    // As the reference's select kernels: flag = (PRED) ? 1 : 0 with the
+    // element named x (Algorithm.cpp:215-237).
+    s += "struct user_pred\n{\n"
+         "    static __device__ __forceinline__ bool test(T x)\n"
+         "    {\n"
+         "        return (\n#line 1 \"predicate\"\n";
+    s += predicate;
+    s += "\n#line 1 \"clc_rtc_kernels\"\n) ? true : false;\n    }\n};\n"
+         "} // namespace clc_ocl\n";
+    s += "extern \"C\" __global__\n"
+         "__launch_bounds__(clc::dev::SCAN_BLOCK) void clc_rtc_copy_if(\n"
+         "    const clc_ocl::T* in, clc_ocl::T* out, size_t n, int vec_ok,\n"
+         "    uint64_t* status, uint32_t* tile_counter, uint32_t* count)\n{\n"
+         "    clc::dev::select_tile<clc_ocl::T, clc_ocl::user_pred, clc::dev::SCAN_BLOCK,\n"
+         "                          clc::dev::scan_items<clc_ocl::T>()>(\n"
+         "        in, out, n, vec_ok, status, tile_counter, count);\n}\n";
+    return s;
+}
+
+std::string operator_source(int type, const std::string& op,
+                            const std::string& identity, int kernels)
+{
+    std::string s = source_head(type);
     // The expressions sit on lines of their own so the build log points at
     // them by name ("operator:1:7: error: ...").
     s += "struct user_op\n{\n"

@@ -54,6 +54,13 @@ enum kernel_set : int
 std::string operator_source(int type, const std::string& op,
                             const std::string& identity, int kernels);
 
+//! Source of the copy_if kernel over elements of type with the predicate
+//! expression predicate over x (OpenCL C scalar syntax):
+//!   clc_rtc_copy_if(const T* in, T* out, size_t n, int vec_ok,
+//!                   uint64_t* status, uint32_t* counter, uint32_t* count)
+std::string predicate_source(int type, const std::string& predicate);
+constexpr const char* COPY_IF_KERNEL = "clc_rtc_copy_if";
+
 //! Compiles source with hipRTC for arch (a gfx target name such as
 //! gfx950); returns the code object. Throws build_error with the log.
 std::vector<char> compile(const std::string& source, const std::string& arch);

@@ -318,5 +318,41 @@ void reduce_expr(context& ctx, const void* in, std::size_t n, int type,
     std::memcpy(result, ctx.pinned(), es);
 }
 
+std::size_t copy_if(context& ctx, const void* in, void* out, std::size_t n, int type,
+                    const char* predicate)
+{
+    const std::size_t es = type_size(type);
+    if (predicate == nullptr) throw std::invalid_argument("clc_copy_if: null predicate");
+    if (n == 0) return 0;
+    check_size(n);
+    if (in == nullptr || out == nullptr)
+        throw std::invalid_argument("clc_copy_if: null buffer");
+    ctx.activate();
+    hipFunction_t f = ctx.programs().function(rtc::predicate_source(type, predicate),
+                                              rtc::COPY_IF_KERNEL);
+    const std::size_t tile =
+        std::size_t(SCAN_BLOCK) * scan_items_of(static_cast<unsigned>(es));
+    const std::size_t tiles = (n + tile - 1) / tile;
+    int vec_ok = reinterpret_cast<uintptr_t>(in) % 16 == 0 ? 1 : 0;
+    const std::size_t bytes = 256 + tiles * 16;
+    temp_buffer tmp(ctx.resource(), bytes);
+    char* t = tmp.as<char>();
+    detail::zero_fill(ctx.stream(), t, bytes);
+    void* status = t + 256;
+    void* counter = t;
+    void* count = t + 64;
+    void* args[] = { &in, &out, &n, &vec_ok, &status, &counter, &count };
+    {
+        timed_launch tl(ctx, "rtc_copy_if");
+        CLC_HIP(hipModuleLaunchKernel(f, static_cast<unsigned>(tiles), 1, 1, SCAN_BLOCK,
+                                      1, 1, 0, ctx.stream(), args, nullptr));
+    }
+    CLC_HIP(hipMemcpyAsync(ctx.pinned(), count, 4, hipMemcpyDeviceToHost, ctx.stream()));
+    CLC_HIP(hipStreamSynchronize(ctx.stream()));
+    uint32_t kept = 0;
+    std::memcpy(&kept, ctx.pinned(), 4);
+    return kept;
+}
+
 } // namespace detail
 } // namespace clc

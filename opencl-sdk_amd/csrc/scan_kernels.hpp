@@ -1,5 +1,6 @@
-// Scan and reduce kernel bodies for gfx950, templated on the element type T
-// and an operator functor Op (static __device__ T apply(T a, T b)).
+// Scan, reduce and copy_if kernel bodies for gfx950, templated on the
+// element type T and an operator functor Op (static __device__ T apply(T a,
+// T b)) or a predicate functor Pred (static __device__ bool test(T x)).
 //
 // The same bodies serve the canonical operators, compiled ahead of time
 // (scan.hip), and any other operator expression, compiled at run time by
@@ -357,6 +358,139 @@ __device__ __forceinline__ void scan_tile(
                 if (i < n) out[i] = y[j * VEC + e];
             }
     }
+}
+
+// ---- copy_if (stream compaction) -----------------------------------------------
+
+struct count_plus
+{
+    static __device__ __forceinline__ uint32_t apply(uint32_t a, uint32_t b) { return a + b; }
+};
+
+//! One tile of a single-pass, order-preserving copy_if: the elements x of
+//! the tile with Pred::test(x) are written to out at their rank among all
+//! such elements of in[0, n). Same tiles, layout, status words and
+//! look-back as scan_tile, over kept-element counts. Each span piece ranks
+//! its kept elements with one ballot per element position; the stores of a
+//! wave land in one dense output run (the L2 combines them into full
+//! lines). The last tile writes the total count to *count. out may alias
+//! in: an element moves only to a lower or equal index, after every
+//! earlier tile has read its input (a tile's aggregate is published after
+//! its loads).
+template <typename T, typename Pred, int BLOCK, int ITEMS, int WIN = 16>
+__device__ __forceinline__ void select_tile(
+    const T* in, T* out, size_t n, int vec_ok, uint64_t* status,
+    uint32_t* tile_counter, uint32_t* count)
+{
+    constexpr int W = BLOCK / WAVE;
+    constexpr int VEC = 16 / sizeof(T);
+    static_assert(ITEMS % VEC == 0, "items per thread");
+    constexpr int J = ITEMS / VEC;
+    constexpr size_t SPAN = size_t(WAVE) * ITEMS;
+    constexpr size_t TILE = size_t(BLOCK) * ITEMS;
+    __shared__ uint32_t s_wave[W];
+    __shared__ uint32_t s_prefix;
+    __shared__ uint32_t s_tile;
+
+    const uint32_t tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
+    if (tid == 0) s_tile = fetch_add_agent(tile_counter, 1u);
+    __syncthreads();
+    const size_t tile = s_tile;
+    const size_t first = tile * TILE + wave * SPAN + size_t(lane) * VEC;
+    const bool full = (tile + 1) * TILE <= n;
+
+    T x[ITEMS];
+    if (full && vec_ok)
+    {
+        const uint4* src = reinterpret_cast<const uint4*>(in + first);
+#pragma unroll
+        for (int j = 0; j < J; ++j)
+        {
+            const uint4 q = src[j * WAVE];
+            __builtin_memcpy(&x[j * VEC], &q, 16);
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int j = 0; j < J; ++j)
+#pragma unroll
+            for (int e = 0; e < VEC; ++e)
+            {
+                const size_t i = first + size_t(j) * WAVE * VEC + e;
+                x[j * VEC + e] = (full || i < n) ? in[i] : T{};
+            }
+    }
+
+    // Rank of every kept element within the wave's span (wave-uniform piece
+    // bases; lanes below and earlier positions of the lane before it).
+    uint32_t pos[ITEMS];
+    uint32_t keep_bits = 0; // bit j*VEC+e: element kept (ITEMS <= 32)
+    static_assert(ITEMS <= 32, "keep bits");
+    uint32_t base = 0;
+    const uint64_t below = (uint64_t(1) << lane) - 1;
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+    {
+        bool f[VEC];
+        uint32_t before = 0, total = 0;
+#pragma unroll
+        for (int e = 0; e < VEC; ++e)
+        {
+            const size_t i = first + size_t(j) * WAVE * VEC + e;
+            f[e] = (full || i < n) && Pred::test(x[j * VEC + e]);
+            const uint64_t m = __ballot(f[e]);
+            before += __builtin_popcountll(m & below);
+            total += __builtin_popcountll(m);
+        }
+        uint32_t own = base + before;
+#pragma unroll
+        for (int e = 0; e < VEC; ++e)
+        {
+            pos[j * VEC + e] = own;
+            if (f[e])
+            {
+                keep_bits |= 1u << (j * VEC + e);
+                ++own;
+            }
+        }
+        base += total;
+    }
+    if (lane == 0) s_wave[wave] = base;
+    __syncthreads();
+    uint32_t wave_prefix = 0, block_total = 0;
+    for (int w = 0; w < W; ++w)
+    {
+        if (w < static_cast<int>(wave)) wave_prefix += s_wave[w];
+        block_total += s_wave[w];
+    }
+
+    if (wave == 0)
+    {
+        uint64_t* mine = status + 2 * tile;
+        uint32_t prefix = 0;
+        if (tile == 0)
+        {
+            if (lane == 0) publish_status(mine, block_total, SCAN_FLAG_PREFIX);
+        }
+        else
+        {
+            if (lane == 0) publish_status(mine, block_total, SCAN_FLAG_AGG);
+            prefix = window_lookback<uint32_t, count_plus, WIN>(status, tile);
+            if (lane == 0) publish_status(mine, prefix + block_total, SCAN_FLAG_PREFIX);
+        }
+        if (lane == 0)
+        {
+            s_prefix = prefix;
+            if ((tile + 1) * TILE >= n) *count = prefix + block_total;
+        }
+    }
+    __syncthreads();
+
+    const size_t off = size_t(s_prefix) + wave_prefix;
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k)
+        if (keep_bits & (1u << k)) out[off + pos[k]] = x[k];
 }
 
 // ---- reduce --------------------------------------------------------------------

@@ -267,6 +267,25 @@ def reduce_fn(x: np.ndarray, fn, identity, init):
     return conv(fn(conv(init), acc))
 
 
+def copy_if(x: np.ndarray, predicate) -> tuple[np.ndarray, int]:
+    """The reference's copy_if (Algorithm.cpp:529-567): flags[i] =
+    predicate(x[i]) ? 1 : 0, positions = exclusive scan of the flags (with
+    uint32 '+'), out[positions[i]] = x[i] for every flagged i, count =
+    positions[n-1] + flags[n-1]. predicate maps a numpy array to a boolean
+    array. Returns (out[:count], count)."""
+    x = np.ascontiguousarray(x)
+    n = x.size
+    if n == 0:
+        return x[:0].copy(), 0
+    flags = np.asarray(predicate(x), dtype=bool).astype(np.uint32)
+    positions = scan(flags, "plus", 0, exclusive=True)
+    count = int(positions[-1]) + int(flags[-1])
+    out = np.zeros(n, dtype=x.dtype)
+    sel = flags.astype(bool)
+    out[positions[sel]] = x[sel]
+    return out[:count], count
+
+
 def histogram_even(x: np.ndarray, bins: int, lower: float, upper: float) -> np.ndarray:
     """Even-width histogram with the reference's float binning
     (Algorithm.cpp:240-273, :569-615)."""

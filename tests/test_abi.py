@@ -155,3 +155,29 @@ def test_float_builtins_compile(t, op, ident):
     L = clc_hip.lib()
     assert L.clc_operator_supported(t, op.encode(), ident.encode()) == clc_hip.SUCCESS, \
         L.clc_last_error().decode()[-2000:]
+
+
+@pytest.mark.parametrize("t,pred,status", [
+    (clc_hip.U32, "(x % 3u) == 0u", clc_hip.SUCCESS),  # main.cpp:321
+    (clc_hip.I32, "x > 0", clc_hip.SUCCESS),  # capi.c:130
+    (clc_hip.F32, "x < 0.5f && !isnan(x)", clc_hip.SUCCESS),
+    (clc_hip.U8, "x & 1", clc_hip.SUCCESS),
+    (clc_hip.F64, "fabs(x) > 1.0", clc_hip.SUCCESS),
+    (clc_hip.I64, "popcount(x) > 32", clc_hip.SUCCESS),
+    (clc_hip.U32, "x +* y", clc_hip.ERROR_BUILD),
+    (clc_hip.F32, "x & 1", clc_hip.ERROR_BUILD),
+    (42, "x > 0", clc_hip.ERROR_INVALID_ARGUMENT),
+])
+def test_copy_if_predicates_compile_or_fail(t, pred, status):
+    L = clc_hip.lib()
+    assert L.clc_predicate_supported(t, pred.encode()) == status, \
+        L.clc_last_error().decode()[-2000:]
+    if status == clc_hip.ERROR_BUILD:
+        assert b"build log" in L.clc_last_error() and b"predicate:1:" in L.clc_last_error()
+
+
+def test_copy_if_null_arguments():
+    L = clc_hip.lib()
+    c = ctypes.c_size_t(7)
+    assert L.clc_copy_if(None, None, None, 10, clc_hip.U32, b"x > 0", ctypes.byref(c)) \
+        == clc_hip.ERROR_INVALID_ARGUMENT

@@ -273,6 +273,38 @@ def test_scan_fn_non_commutative_order_closed_form():
     assert not np.array_equal(swapped, out)
 
 
+# ---- copy_if -------------------------------------------------------------------
+
+@pytest.mark.parametrize("n", SIZES)
+def test_copy_if_reference_test_rule(golden, n):
+    """main.cpp:316-329: copy_if(x, "(x % 3u) == 0u") on the histogram test's
+    inputs equals the in-order list of multiples of 3."""
+    x = golden[f"hist_in_{n}"]
+    out, count = oracle.copy_if(x, lambda v: v % 3 == 0)
+    ref = [int(v) for v in x if v % 3 == 0]
+    assert count == len(ref)
+    assert out.tolist() == ref
+
+
+def test_copy_if_capi_vector(golden):
+    """capi.c:130-131: copy_if(x > 0) keeps every positive element."""
+    x = golden["capi_in"]
+    out, count = oracle.copy_if(x, lambda v: v > 0)
+    assert count == int((x > 0).sum())
+    assert out.tolist() == [int(v) for v in x if v > 0]
+
+
+def test_copy_if_edges():
+    x = np.arange(10, dtype=np.int64)
+    assert oracle.copy_if(x, lambda v: v < 0)[1] == 0
+    out, c = oracle.copy_if(x, lambda v: v >= 0)
+    assert c == 10 and out.tolist() == list(range(10))
+    assert oracle.copy_if(x[:0], lambda v: v > 0)[1] == 0
+    f = np.array([-0.0, 0.0, np.nan, 1.5, -2.0], dtype=np.float32)
+    out, c = oracle.copy_if(f, lambda v: v <= 0)
+    assert c == 3 and np.signbit(out[0]) and not np.signbit(out[1]) and out[2] == -2.0
+
+
 # ---- histogram ---------------------------------------------------------------
 
 @pytest.mark.parametrize("n", SIZES)
