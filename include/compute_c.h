@@ -193,6 +193,29 @@ clc_status clc_scan(clc_context ctx, const void* in, void* out, size_t n,
                     clc_type type, const char* op, const char* identity,
                     int exclusive);
 
+/* out[i] = value for i < n (value: one host element of type).
+ * (reference clc_fill, compute_c.h:142-143; Algorithm.cpp:331-343) */
+clc_status clc_fill(clc_context ctx, void* out, size_t n, clc_type type, const void* value);
+
+/* out[i] = start + (T)i * step, integer arithmetic wrapping modulo 2^bits of
+ * the type; floating point as a product then a sum, each rounded.
+ * (reference clc_sequence, compute_c.h:144-146; Algorithm.cpp:345-359) */
+clc_status clc_sequence(clc_context ctx, void* out, size_t n, clc_type type,
+                        const void* start, const void* step);
+
+/* out[i] = (T_OUT)(expression) with x = in[i] (an OpenCL C expression; the
+ * types are available as T_IN and T_OUT). Compiled once per context;
+ * CLC_ERROR_BUILD with the build log when it does not compile.
+ * (reference clc_transform, compute_c.h:147-150; Algorithm.cpp:361-377) */
+clc_status clc_transform(clc_context ctx, const void* in, clc_type in_type, void* out,
+                         clc_type out_type, size_t n, const char* expression);
+
+/* The same with x = in1[i], y = in2[i] (types T_IN1, T_IN2, T_OUT).
+ * (reference clc_transform2, compute_c.h:151-155; Algorithm.cpp:379-398) */
+clc_status clc_transform2(clc_context ctx, const void* in1, clc_type type1, const void* in2,
+                          clc_type type2, void* out, clc_type out_type, size_t n,
+                          const char* expression);
+
 /* Copies the elements x of in[0, n) for which predicate (an OpenCL C
  * expression of x, as the reference's) is true to the front of out, in
  * order, and stores how many in *count; blocks until the count is on the
@@ -351,6 +374,12 @@ clc_status clc_learned_segmented_sort(clc_context ctx, void* keys, clc_type key_
  * or CLC_ERROR_INVALID_ARGUMENT. Needs no device. */
 clc_status clc_operator_supported(clc_type type, const char* op,
                                   const char* identity);
+
+/* Whether clc_transform (in2_type < 0) or clc_transform2 accepts expression
+ * for the types: CLC_SUCCESS when it compiles for gfx950, CLC_ERROR_BUILD
+ * with the log otherwise, or CLC_ERROR_INVALID_ARGUMENT. Needs no device. */
+clc_status clc_transform_supported(clc_type in_type, int in2_type, clc_type out_type,
+                                   const char* expression);
 
 /* Whether clc_copy_if accepts predicate for type: CLC_SUCCESS when it
  * compiles for gfx950, CLC_ERROR_BUILD with the log otherwise, or

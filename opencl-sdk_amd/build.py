@@ -33,7 +33,7 @@ COMMON = [
 
 # Headers compiled again at run time by hipRTC (operator expressions of
 # clc_scan / clc_reduce); their text is embedded in the library.
-RTC_HEADERS = ["device_common.hpp", "scan_kernels.hpp"]
+RTC_HEADERS = ["device_common.hpp", "scan_kernels.hpp", "elementwise_kernels.hpp"]
 RTC_DELIM = "CLCHDR"  # raw-string delimiters hold at most 16 characters
 
 

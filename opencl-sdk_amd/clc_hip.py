@@ -92,6 +92,11 @@ SIGNATURES = {
     "clc_context_program_stats": (_i, [_vp, ctypes.POINTER(ProgramStats)]),
     "clc_copy_if": (_i, [_vp, _vp, _vp, _sz, _i, _cp, ctypes.POINTER(_sz)]),
     "clc_predicate_supported": (_i, [_i, _cp]),
+    "clc_fill": (_i, [_vp, _vp, _sz, _i, _vp]),
+    "clc_sequence": (_i, [_vp, _vp, _sz, _i, _vp, _vp]),
+    "clc_transform": (_i, [_vp, _vp, _i, _vp, _i, _sz, _cp]),
+    "clc_transform2": (_i, [_vp, _vp, _i, _vp, _i, _vp, _i, _sz, _cp]),
+    "clc_transform_supported": (_i, [_i, _i, _i, _cp]),
     "clc_context_set_profiling": (_i, [_vp, _i]),
     "clc_context_kernel_timings": (_i, [_vp, ctypes.POINTER(KernelTiming), _sz,
                                         ctypes.POINTER(_sz)]),
@@ -138,6 +143,12 @@ def lib() -> ctypes.CDLL:
 def check(status: int) -> None:
     if status != SUCCESS:
         raise ClcError(status, lib().clc_last_error().decode(errors="replace"))
+
+
+def _np_of():
+    import numpy as np
+    return {I8: np.int8, U8: np.uint8, I16: np.int16, U16: np.uint16, I32: np.int32,
+            U32: np.uint32, I64: np.int64, U64: np.uint64, F32: np.float32, F64: np.float64}
 
 
 def _enc(s):
@@ -334,6 +345,42 @@ class Context:
                                ini.ctypes.data_as(ctypes.c_void_p),
                                res.ctypes.data_as(ctypes.c_void_p)))
         return res[0]
+
+    def _one(self, t, v):
+        import numpy as np
+        return np.array([v], dtype=_np_of()[t])
+
+    def fill(self, out, value, type_code=None, n=None):
+        t = dtype_code(out.dtype) if type_code is None else type_code
+        n = out.numel() if n is None else n
+        v = self._one(t, value)
+        check(lib().clc_fill(self.handle, ctypes.c_void_p(out.data_ptr()), n, t,
+                             v.ctypes.data_as(ctypes.c_void_p)))
+
+    def sequence(self, out, start=0, step=1, type_code=None, n=None):
+        t = dtype_code(out.dtype) if type_code is None else type_code
+        n = out.numel() if n is None else n
+        a, b = self._one(t, start), self._one(t, step)
+        check(lib().clc_sequence(self.handle, ctypes.c_void_p(out.data_ptr()), n, t,
+                                 a.ctypes.data_as(ctypes.c_void_p),
+                                 b.ctypes.data_as(ctypes.c_void_p)))
+
+    def transform(self, inp, out, expression, in_type=None, out_type=None, n=None):
+        ti = dtype_code(inp.dtype) if in_type is None else in_type
+        to = dtype_code(out.dtype) if out_type is None else out_type
+        n = inp.numel() if n is None else n
+        check(lib().clc_transform(self.handle, ctypes.c_void_p(inp.data_ptr()), ti,
+                                  ctypes.c_void_p(out.data_ptr()), to, n, _enc(expression)))
+
+    def transform2(self, in1, in2, out, expression, type1=None, type2=None, out_type=None,
+                   n=None):
+        t1 = dtype_code(in1.dtype) if type1 is None else type1
+        t2 = dtype_code(in2.dtype) if type2 is None else type2
+        to = dtype_code(out.dtype) if out_type is None else out_type
+        n = in1.numel() if n is None else n
+        check(lib().clc_transform2(self.handle, ctypes.c_void_p(in1.data_ptr()), t1,
+                                   ctypes.c_void_p(in2.data_ptr()), t2,
+                                   ctypes.c_void_p(out.data_ptr()), to, n, _enc(expression)))
 
     def copy_if(self, inp, out, predicate, type_code=None, n=None) -> int:
         """clc_copy_if: elements of inp satisfying predicate (an expression

@@ -388,6 +388,58 @@ clc_status clc_copy_if(clc_context ctx, const void* in, void* out, size_t n,
     });
 }
 
+clc_status clc_fill(clc_context ctx, void* out, size_t n, clc_type type, const void* value)
+{
+    return guarded([&] {
+        require(ctx != nullptr && value != nullptr, "clc: null argument");
+        clc::detail::fill(get(ctx), out, n, type, value);
+    });
+}
+
+clc_status clc_sequence(clc_context ctx, void* out, size_t n, clc_type type,
+                        const void* start, const void* step)
+{
+    return guarded([&] {
+        require(ctx != nullptr && start != nullptr && step != nullptr, "clc: null argument");
+        clc::detail::sequence(get(ctx), out, n, type, start, step);
+    });
+}
+
+clc_status clc_transform(clc_context ctx, const void* in, clc_type in_type, void* out,
+                         clc_type out_type, size_t n, const char* expression)
+{
+    return guarded([&] {
+        require(ctx != nullptr && expression != nullptr, "clc: null argument");
+        clc::detail::transform(get(ctx), in, in_type, out, out_type, n, expression);
+    });
+}
+
+clc_status clc_transform2(clc_context ctx, const void* in1, clc_type type1, const void* in2,
+                          clc_type type2, void* out, clc_type out_type, size_t n,
+                          const char* expression)
+{
+    return guarded([&] {
+        require(ctx != nullptr && expression != nullptr, "clc: null argument");
+        clc::detail::transform2(get(ctx), in1, type1, in2, type2, out, out_type, n,
+                                expression);
+    });
+}
+
+clc_status clc_transform_supported(clc_type in_type, int in2_type, clc_type out_type,
+                                   const char* expression)
+{
+    return guarded([&] {
+        require(expression != nullptr, "clc: null argument");
+        clc::type_size(in_type);
+        clc::type_size(out_type);
+        if (in2_type >= 0) clc::type_size(in2_type);
+        (void)clc::rtc::compile(
+            clc::rtc::transform_source(in_type, in2_type < 0 ? -1 : in2_type, out_type,
+                                       expression),
+            "gfx950");
+    });
+}
+
 clc_status clc_predicate_supported(clc_type type, const char* predicate)
 {
     return guarded([&] {

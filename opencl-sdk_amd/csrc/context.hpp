@@ -227,6 +227,15 @@ void reduce_expr(context& ctx, const void* in, std::size_t n, int type,
 //! until the count is on the host). out may alias in.
 std::size_t copy_if(context& ctx, const void* in, void* out, std::size_t n, int type,
                     const char* predicate);
+//! Elementwise helpers (elementwise.hip): value / start / step point to one
+//! host element of type; expressions are OpenCL C over x (and y).
+void fill(context& ctx, void* out, std::size_t n, int type, const void* value);
+void sequence(context& ctx, void* out, std::size_t n, int type, const void* start,
+              const void* step);
+void transform(context& ctx, const void* in, int in_type, void* out, int out_type,
+               std::size_t n, const char* expression);
+void transform2(context& ctx, const void* in1, int type1, const void* in2, int type2,
+                void* out, int out_type, std::size_t n, const char* expression);
 void histogram_even(context& ctx, const void* in, std::size_t n, int type,
                     uint32_t* histogram, std::size_t num_bins, double lower,
                     double upper);

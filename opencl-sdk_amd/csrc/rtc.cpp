@@ -318,20 +318,71 @@ std::string base_arch(const std::string& arch)
 
 namespace {
 
-// Headers, the builtins prelude and "typedef <element type> T;" opening
-// namespace clc_ocl.
-std::string source_head(int type)
+// Headers and the builtins prelude, leaving namespace clc_ocl open.
+std::string source_open()
 {
     std::string s;
     s += "#include <cstdint>\n#include <type_traits>\n";
-    s += "#include \"scan_kernels.hpp\"\n";
+    s += "#include \"scan_kernels.hpp\"\n#include \"elementwise_kernels.hpp\"\n";
     s += PRELUDE;
     s += "namespace clc_ocl {\n";
-    s += std::string("typedef ") + type_name(type) + " T;\n";
     return s;
 }
 
+// The same with "typedef <element type> T;".
+std::string source_head(int type)
+{
+    return source_open() + "typedef " + type_name(type) + " T;\n";
+}
+
+// "return (OUT)(<expression>);" with the expression on lines of its own,
+// named name in the build log.
+std::string return_expression(const char* out_type, const std::string& expr,
+                              const char* name)
+{
+    return std::string("        return (") + out_type + ")(\n#line 1 \"" + name + "\"\n"
+        + expr + "\n#line 1 \"clc_rtc_kernels\"\n);\n";
+}
+
 } // namespace
+
+std::string transform_source(int in_type, int in2_type, int out_type,
+                             const std::string& expression)
+{
+    // As the reference's transform kernels: T_IN x (T_IN1 x, T_IN2 y) in,
+    // (T_OUT)(EXPR) out (Algorithm.cpp:30-56).
+    const bool two = in2_type >= 0;
+    std::string s = source_open();
+    s += std::string("typedef ") + type_name(out_type) + " T_OUT;\n";
+    if (two)
+        s += std::string("typedef ") + type_name(in_type) + " T_IN1;\ntypedef "
+            + type_name(in2_type) + " T_IN2;\n"
+            + "struct user_expr\n{\n"
+              "    static __device__ __forceinline__ T_OUT apply(T_IN1 x, T_IN2 y)\n    {\n"
+            + return_expression("T_OUT", expression, "expression") + "    }\n};\n";
+    else
+        s += std::string("typedef ") + type_name(in_type) + " T_IN;\n"
+            + "struct user_expr\n{\n"
+              "    static __device__ __forceinline__ T_OUT apply(T_IN x)\n    {\n"
+            + return_expression("T_OUT", expression, "expression") + "    }\n};\n";
+    s += "} // namespace clc_ocl\n";
+    if (two)
+        s += "extern \"C\" __global__\n"
+             "__launch_bounds__(clc::dev::EW_BLOCK) void clc_rtc_transform2(\n"
+             "    const clc_ocl::T_IN1* in1, const clc_ocl::T_IN2* in2, clc_ocl::T_OUT* out,\n"
+             "    size_t n)\n{\n"
+             "    clc::dev::transform2_tile<clc_ocl::T_IN1, clc_ocl::T_IN2, clc_ocl::T_OUT,\n"
+             "                              clc_ocl::user_expr, clc::dev::EW_BLOCK,\n"
+             "                              clc::dev::EW_ITEMS>(in1, in2, out, n);\n}\n";
+    else
+        s += "extern \"C\" __global__\n"
+             "__launch_bounds__(clc::dev::EW_BLOCK) void clc_rtc_transform(\n"
+             "    const clc_ocl::T_IN* in, clc_ocl::T_OUT* out, size_t n)\n{\n"
+             "    clc::dev::transform_tile<clc_ocl::T_IN, clc_ocl::T_OUT, clc_ocl::user_expr,\n"
+             "                             clc::dev::EW_BLOCK, clc::dev::EW_ITEMS>(\n"
+             "        in, out, n);\n}\n";
+    return s;
+}
 
 std::string predicate_source(int type, const std::string& predicate)
 {

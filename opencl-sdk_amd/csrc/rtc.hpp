@@ -61,6 +61,15 @@ std::string operator_source(int type, const std::string& op,
 std::string predicate_source(int type, const std::string& predicate);
 constexpr const char* COPY_IF_KERNEL = "clc_rtc_copy_if";
 
+//! Source of the transform kernel (in2_type < 0: out[i] = (T_OUT)(expr)
+//! with x = in[i]) or the transform2 kernel (x = in1[i], y = in2[i]):
+//!   clc_rtc_transform(const T_IN* in, T_OUT* out, size_t n)
+//!   clc_rtc_transform2(const T_IN1* in1, const T_IN2* in2, T_OUT* out, size_t n)
+std::string transform_source(int in_type, int in2_type, int out_type,
+                             const std::string& expression);
+constexpr const char* TRANSFORM_KERNEL = "clc_rtc_transform";
+constexpr const char* TRANSFORM2_KERNEL = "clc_rtc_transform2";
+
 //! Compiles source with hipRTC for arch (a gfx target name such as
 //! gfx950); returns the code object. Throws build_error with the log.
 std::vector<char> compile(const std::string& source, const std::string& arch);

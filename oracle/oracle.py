@@ -267,6 +267,27 @@ def reduce_fn(x: np.ndarray, fn, identity, init):
     return conv(fn(conv(init), acc))
 
 
+def sequence(n: int, dtype, start, step) -> np.ndarray:
+    """The reference's sequence: out[i] = start + (T)i * step
+    (Algorithm.cpp:23-27, :345-359). Integers wrap modulo 2^bits of T (the
+    OpenCL C conversion of the promoted result); floating point rounds the
+    product, then the sum."""
+    dt = np.dtype(dtype)
+    i = np.arange(n, dtype=np.uint64)
+    if dt.kind == "f":
+        ti = i.astype(np.uint32).astype(dt)  # (T)i from the uint index
+        return (ti * dt.type(step)).astype(dt) + dt.type(start)
+    bits = dt.itemsize * 8
+    mask = np.uint64((1 << bits) - 1)
+    s = np.uint64(int(start) & int(mask))
+    d = np.uint64(int(step) & int(mask))
+    ti = i & mask
+    with np.errstate(over="ignore"):
+        v = (s + ti * d) & mask
+    return v.astype(dt.newbyteorder("=")) if dt.kind == "u" else v.astype(
+        np.dtype(f"u{dt.itemsize}")).view(dt)
+
+
 def copy_if(x: np.ndarray, predicate) -> tuple[np.ndarray, int]:
     """The reference's copy_if (Algorithm.cpp:529-567): flags[i] =
     predicate(x[i]) ? 1 : 0, positions = exclusive scan of the flags (with

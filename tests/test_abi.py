@@ -181,3 +181,22 @@ def test_copy_if_null_arguments():
     c = ctypes.c_size_t(7)
     assert L.clc_copy_if(None, None, None, 10, clc_hip.U32, b"x > 0", ctypes.byref(c)) \
         == clc_hip.ERROR_INVALID_ARGUMENT
+
+
+@pytest.mark.parametrize("ti,t2,to,expr,status", [
+    (clc_hip.I32, -1, clc_hip.F32, "(float)x * 0.5f", clc_hip.SUCCESS),  # capi.c:124
+    (clc_hip.U32, -1, clc_hip.F32, "x * 0.18f + 32.0f", clc_hip.SUCCESS),  # compute-pipeline
+    (clc_hip.U32, clc_hip.U32, clc_hip.U32, "x + y", clc_hip.SUCCESS),  # compute-pipeline
+    (clc_hip.F64, clc_hip.I8, clc_hip.I64, "convert_long(x) * y + (T_OUT)sizeof(T_IN2)",
+     clc_hip.SUCCESS),
+    (clc_hip.U8, -1, clc_hip.U16, "popcount(x) + clz(x)", clc_hip.SUCCESS),
+    (clc_hip.I32, -1, clc_hip.I32, "x +* nonsense", clc_hip.ERROR_BUILD),
+    (clc_hip.I32, -1, clc_hip.I32, "x + y", clc_hip.ERROR_BUILD),  # no y in transform
+    (clc_hip.I32, 42, clc_hip.I32, "x + y", clc_hip.ERROR_INVALID_ARGUMENT),
+])
+def test_transform_expressions_compile_or_fail(ti, t2, to, expr, status):
+    L = clc_hip.lib()
+    assert L.clc_transform_supported(ti, t2, to, expr.encode()) == status, \
+        L.clc_last_error().decode()[-2000:]
+    if status == clc_hip.ERROR_BUILD:
+        assert b"expression:1:" in L.clc_last_error()

@@ -273,6 +273,30 @@ def test_scan_fn_non_commutative_order_closed_form():
     assert not np.array_equal(swapped, out)
 
 
+# ---- sequence -------------------------------------------------------------------
+
+def test_sequence_capi_closed_form():
+    """capi.c:139-141: sequence(start 5, step 2) gives 5 + 2i."""
+    n = 100000
+    out = oracle.sequence(n, np.int32, 5, 2)
+    assert out[0] == 5 and out[-1] == 5 + 2 * (n - 1)
+    np.testing.assert_array_equal(out, 5 + 2 * np.arange(n, dtype=np.int64))
+
+
+def test_sequence_wraps_and_rounds():
+    u8 = oracle.sequence(600, np.uint8, 250, 3)
+    assert u8.tolist() == [(250 + 3 * i) % 256 for i in range(600)]
+    i8 = oracle.sequence(300, np.int8, -128, -1)
+    assert i8.tolist() == [((-128 - i + 128) % 256) - 128 for i in range(300)]
+    i64 = oracle.sequence(5, np.int64, 2**63 - 2, 1)
+    assert i64.tolist() == [2**63 - 2, 2**63 - 1, -2**63, -2**63 + 1, -2**63 + 2]
+    f = oracle.sequence(1 << 25, np.float32, 0.5, 1.0)
+    i = np.arange(1 << 25, dtype=np.float64)
+    want = np.float32(np.float32(i) * np.float32(1.0)) + np.float32(0.5)
+    np.testing.assert_array_equal(f, want.astype(np.float32))
+    assert f[(1 << 24) + 1] == np.float32((1 << 24) + 0.5)  # (T)i rounds to 2^24
+
+
 # ---- copy_if -------------------------------------------------------------------
 
 @pytest.mark.parametrize("n", SIZES)
