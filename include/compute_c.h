@@ -5,8 +5,10 @@
  * lib/include/CL/Compute/compute_c.h).
  *
  * Differences from the reference boundary, all deliberate:
- *   - clc_buffer handles are replaced by HIP device pointers (void*), and
- *     the command queue by a hipStream_t bound to the context at creation
+ *   - The algorithms take HIP device pointers (void*) where the reference
+ *     takes clc_buffer handles; the handles remain (clc_buffer_*) as
+ *     library-owned memory, passed to the algorithms as clc_buffer_data().
+ *     The command queue is a hipStream_t bound to the context at creation
  *     (clc_context_create_from_stream). Bounds cannot be checked on raw
  *     pointers: the caller guarantees that n elements fit.
  *   - clc_last_cl_error() returns the hipError_t of the last
@@ -150,6 +152,37 @@ typedef struct clc_program_stats
     size_t builds;
 } clc_program_stats;
 clc_status clc_context_program_stats(clc_context ctx, clc_program_stats* stats);
+
+/* ---- buffers (replaces compute_c.h:100-117) ----------------------------- */
+/* Library-owned device memory for callers that do not manage their own:
+ * a block of the context's pool. The algorithms take device pointers, so a
+ * buffer is passed to them as clc_buffer_data(buffer). The block may outlive
+ * the context (it keeps the pool alive). */
+typedef struct clc_buffer_s* clc_buffer;
+
+/* bytes may be 0: the handle is valid and holds no device memory.
+ * (reference clc_buffer_alloc, compute_c.h:103-104, CApi.cpp:301-312) */
+clc_status clc_buffer_alloc(clc_context ctx, size_t bytes, clc_buffer* out);
+/* Returns the block to the pool; NULL is a no-op. Work already queued on
+ * the context's stream may still use it: the pool hands it out again only
+ * to work queued after. (reference compute_c.h:105, CApi.cpp:314-320) */
+void clc_buffer_release(clc_buffer buffer);
+/* (reference compute_c.h:106, CApi.cpp:322-325) */
+size_t clc_buffer_size(clc_buffer buffer);
+/* The HIP device pointer of the buffer (NULL for a 0-byte buffer). */
+void* clc_buffer_data(clc_buffer buffer);
+/* Blocking transfers on the context's stream; offset and bytes must stay
+ * within the buffer (CLC_ERROR_INVALID_ARGUMENT otherwise).
+ * (reference compute_c.h:109-114, CApi.cpp:327-355) */
+clc_status clc_buffer_write(clc_context ctx, clc_buffer buffer, size_t offset, size_t bytes,
+                            const void* src);
+clc_status clc_buffer_read(clc_context ctx, clc_buffer buffer, size_t offset, size_t bytes,
+                           void* dst);
+/* Device-to-device copy queued on the context's stream (not blocking, like
+ * the reference's enqueueCopyBuffer). (reference compute_c.h:115-117,
+ * CApi.cpp:357-370) */
+clc_status clc_buffer_copy(clc_context ctx, clc_buffer src, size_t src_offset, clc_buffer dst,
+                           size_t dst_offset, size_t bytes);
 
 /* ---- kernel timing ------------------------------------------------------ */
 /* Per-kernel hipEvent timing on the context's stream, off by default (the

@@ -92,6 +92,13 @@ SIGNATURES = {
     "clc_context_program_stats": (_i, [_vp, ctypes.POINTER(ProgramStats)]),
     "clc_copy_if": (_i, [_vp, _vp, _vp, _sz, _i, _cp, ctypes.POINTER(_sz)]),
     "clc_predicate_supported": (_i, [_i, _cp]),
+    "clc_buffer_alloc": (_i, [_vp, _sz, ctypes.POINTER(_vp)]),
+    "clc_buffer_release": (None, [_vp]),
+    "clc_buffer_size": (_sz, [_vp]),
+    "clc_buffer_data": (_vp, [_vp]),
+    "clc_buffer_write": (_i, [_vp, _vp, _sz, _sz, _vp]),
+    "clc_buffer_read": (_i, [_vp, _vp, _sz, _sz, _vp]),
+    "clc_buffer_copy": (_i, [_vp, _vp, _sz, _vp, _sz, _sz]),
     "clc_fill": (_i, [_vp, _vp, _sz, _i, _vp]),
     "clc_sequence": (_i, [_vp, _vp, _sz, _i, _vp, _vp]),
     "clc_transform": (_i, [_vp, _vp, _i, _vp, _i, _sz, _cp]),

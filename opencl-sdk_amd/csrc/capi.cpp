@@ -28,6 +28,16 @@ struct clc_graph_s
     std::unique_ptr<clc::seg::graph> g;
 };
 
+// A block of the context's memory resource (the pool) and its size; the
+// resource is shared so the block can outlive the context, as in the
+// reference's clc_buffer_s (CApi.cpp:21-26).
+struct clc_buffer_s
+{
+    std::shared_ptr<clc::memory_resource> resource;
+    void* data;
+    std::size_t bytes;
+};
+
 namespace {
 
 thread_local std::string last_error;
@@ -375,6 +385,89 @@ clc_status clc_scan(clc_context ctx, const void* in, void* out, size_t n,
         else
             clc::detail::scan_expr(get(ctx), in, out, n, type, op, identity,
                                    exclusive != 0);
+    });
+}
+
+clc_status clc_buffer_alloc(clc_context ctx, size_t bytes, clc_buffer* out)
+{
+    if (out == nullptr) return fail(CLC_ERROR_INVALID_ARGUMENT, "clc: null argument");
+    *out = nullptr;
+    if (ctx == nullptr) return fail(CLC_ERROR_INVALID_ARGUMENT, "clc: null context");
+    return guarded([&] {
+        clc::context& c = get(ctx);
+        c.activate();
+        std::unique_ptr<clc_buffer_s> b(new clc_buffer_s{ c.resource(), nullptr, bytes });
+        if (bytes != 0) b->data = b->resource->allocate(bytes);
+        *out = b.release();
+    });
+}
+
+void clc_buffer_release(clc_buffer buffer)
+{
+    if (buffer == nullptr) return;
+    if (buffer->bytes != 0) buffer->resource->deallocate(buffer->data, buffer->bytes);
+    delete buffer;
+}
+
+size_t clc_buffer_size(clc_buffer buffer)
+{
+    return buffer == nullptr ? 0 : buffer->bytes;
+}
+
+void* clc_buffer_data(clc_buffer buffer)
+{
+    return buffer == nullptr ? nullptr : buffer->data;
+}
+
+clc_status clc_buffer_write(clc_context ctx, clc_buffer buffer, size_t offset, size_t bytes,
+                            const void* src)
+{
+    return guarded([&] {
+        require(ctx != nullptr && buffer != nullptr, "clc: null argument");
+        require(offset <= buffer->bytes && bytes <= buffer->bytes - offset,
+                "clc: write out of bounds");
+        require(bytes == 0 || src != nullptr, "clc: null source");
+        if (bytes == 0) return;
+        clc::context& c = get(ctx);
+        c.activate();
+        CLC_HIP(hipMemcpyAsync(static_cast<char*>(buffer->data) + offset, src, bytes,
+                               hipMemcpyHostToDevice, c.stream()));
+        CLC_HIP(hipStreamSynchronize(c.stream()));
+    });
+}
+
+clc_status clc_buffer_read(clc_context ctx, clc_buffer buffer, size_t offset, size_t bytes,
+                           void* dst)
+{
+    return guarded([&] {
+        require(ctx != nullptr && buffer != nullptr, "clc: null argument");
+        require(offset <= buffer->bytes && bytes <= buffer->bytes - offset,
+                "clc: read out of bounds");
+        require(bytes == 0 || dst != nullptr, "clc: null destination");
+        if (bytes == 0) return;
+        clc::context& c = get(ctx);
+        c.activate();
+        CLC_HIP(hipMemcpyAsync(dst, static_cast<const char*>(buffer->data) + offset, bytes,
+                               hipMemcpyDeviceToHost, c.stream()));
+        CLC_HIP(hipStreamSynchronize(c.stream()));
+    });
+}
+
+clc_status clc_buffer_copy(clc_context ctx, clc_buffer src, size_t src_offset, clc_buffer dst,
+                           size_t dst_offset, size_t bytes)
+{
+    return guarded([&] {
+        require(ctx != nullptr && src != nullptr && dst != nullptr, "clc: null argument");
+        require(src_offset <= src->bytes && bytes <= src->bytes - src_offset,
+                "clc: copy source out of bounds");
+        require(dst_offset <= dst->bytes && bytes <= dst->bytes - dst_offset,
+                "clc: copy destination out of bounds");
+        if (bytes == 0) return;
+        clc::context& c = get(ctx);
+        c.activate();
+        CLC_HIP(hipMemcpyAsync(static_cast<char*>(dst->data) + dst_offset,
+                               static_cast<const char*>(src->data) + src_offset, bytes,
+                               hipMemcpyDeviceToDevice, c.stream()));
     });
 }
 

@@ -3,6 +3,8 @@ include/compute_c.h declares, and reports errors the way the reference's
 C API does (lib/src/Compute/CApi.cpp:60-115, lib/test/Compute/capi.c)."""
 import ctypes
 import os
+import shutil
+import subprocess
 import re
 
 import pytest
@@ -200,3 +202,28 @@ def test_transform_expressions_compile_or_fail(ti, t2, to, expr, status):
         L.clc_last_error().decode()[-2000:]
     if status == clc_hip.ERROR_BUILD:
         assert b"expression:1:" in L.clc_last_error()
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc")
+def test_header_and_c_test_compile_as_c99(tmp_path):
+    """include/compute_c.h is a C header: the C end-to-end test compiles
+    with -std=c99 -pedantic -Werror and links against the library."""
+    exe = str(tmp_path / "capi_test")
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror", "-O1",
+                    "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cpp", "capi_test.c"), "-o", exe,
+                    "-L", os.path.join(ROOT, "opencl-sdk_amd"), "-lclc_hip", "-lm"],
+                   check=True)
+    assert os.path.exists(exe)
+
+
+def test_buffer_handles_without_device():
+    L = clc_hip.lib()
+    b = ctypes.c_void_p(123)
+    assert L.clc_buffer_alloc(None, 16, ctypes.byref(b)) == clc_hip.ERROR_INVALID_ARGUMENT
+    assert b.value is None
+    assert L.clc_buffer_size(None) == 0
+    assert L.clc_buffer_data(None) is None
+    L.clc_buffer_release(None)
+    assert L.clc_buffer_read(None, None, 0, 0, None) == clc_hip.ERROR_INVALID_ARGUMENT
+    assert L.clc_buffer_copy(None, None, 0, None, 0, 0) == clc_hip.ERROR_INVALID_ARGUMENT
