@@ -91,7 +91,7 @@ void histogram_typed(context& ctx, const void* in, size_t n, uint32_t* hist,
     const size_t per_block = size_t(BLOCK) * 64;
     const uint32_t groups = static_cast<uint32_t>(std::max<size_t>(
         1, std::min<size_t>((n + per_block - 1) / per_block,
-                            size_t(ctx.facts().cu_count) * 4)));
+                            size_t(ctx.facts().cu_count) * 8)));
     const int vec_ok = reinterpret_cast<uintptr_t>(in) % 16 == 0 ? 1 : 0;
     const T* src = static_cast<const T*>(in);
     timed_launch tl(ctx, "histogram_even");
