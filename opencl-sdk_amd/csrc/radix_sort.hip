@@ -95,18 +95,32 @@ __global__ __launch_bounds__(BLOCK) void digit_histogram_kernel(
         constexpr uint32_t PER = 16 / sizeof(K);
         const uint32_t nvec = n / PER;
         const uint4* v = reinterpret_cast<const uint4*>(keys);
-        // Two independent 16-byte loads per thread in flight.
-        for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < nvec; i += 2 * stride)
+        // Two 16-byte loads per thread per round, and the next round's two
+        // issued before this round's LDS atomics (which do not wait on
+        // memory), so four are in flight while the atomics run. Loads past
+        // the end are clamped to the last vector and not counted.
+        uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+        if (nvec != 0 && i < nvec)
         {
-            const uint4 q0 = v[i];
-            const uint4 q1 = v[min(i + stride, nvec - 1)];
-            const K* k0 = reinterpret_cast<const K*>(&q0);
-            const K* k1 = reinterpret_cast<const K*>(&q1);
+            uint4 q0 = v[i];
+            uint4 q1 = v[min(i + stride, nvec - 1)];
+            for (;;)
+            {
+                const uint32_t next = i + 2 * stride;
+                const uint4 r0 = v[min(next, nvec - 1)];
+                const uint4 r1 = v[min(next + stride, nvec - 1)];
+                const K* k0 = reinterpret_cast<const K*>(&q0);
+                const K* k1 = reinterpret_cast<const K*>(&q1);
 #pragma unroll
-            for (uint32_t j = 0; j < PER; ++j) count(k0[j]);
-            if (i + stride < nvec)
+                for (uint32_t j = 0; j < PER; ++j) count(k0[j]);
+                if (i + stride < nvec)
 #pragma unroll
-                for (uint32_t j = 0; j < PER; ++j) count(k1[j]);
+                    for (uint32_t j = 0; j < PER; ++j) count(k1[j]);
+                if (next >= nvec) break;
+                i = next;
+                q0 = r0;
+                q1 = r1;
+            }
         }
         start = nvec * PER;
     }
