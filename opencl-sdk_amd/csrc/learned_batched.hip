@@ -274,8 +274,8 @@ __global__ __launch_bounds__(256) void fit_kernel(const uint32_t* __restrict__ s
         const uint32_t blo = static_cast<uint32_t>(uint64_t(nb) * lo / ns);
         const uint32_t next = l + 1 < LEAVES ? static_cast<uint32_t>(uint64_t(nb) * hi / ns) : nb - 1;
         rmi_leaf& L = sm.leaf[l];
-        L.a = a;
-        L.b = b;
+        L.a = a * double(nb); // positions in buckets, not CDF fractions
+        L.b = b * double(nb);
         L.lo = min(blo, nb - 1);
         L.hi = max(L.lo, min(next, nb - 1));
     }

@@ -18,7 +18,7 @@ constexpr uint32_t SAMPLE_STRIDE = 100;
 
 struct rmi_leaf
 {
-    double a, b; // CDF ~ a * x + b
+    double a, b; // bucket position ~ a * x + b (the CDF times the bucket count)
     uint32_t lo, hi; // bucket range of the leaf
 };
 
@@ -45,22 +45,26 @@ __device__ __forceinline__ double root_feature(const rmi_model& m, uint64_t bits
     return static_cast<double>(m.root_log ? log_image(bits) : bits);
 }
 
+//! Leaf of root feature f: a0 * f + b0 clamped to [0, LEAVES - 1] and
+//! truncated (clamping first keeps the conversion in range).
 __device__ __forceinline__ uint32_t leaf_of(const rmi_model& m, double f)
 {
-    const double r = fma(m.a0, f, m.b0);
-    return r <= 0.0 ? 0u : r >= double(LEAVES - 1) ? LEAVES - 1 : static_cast<uint32_t>(r);
+    const double r = fmin(fmax(fma(m.a0, f, m.b0), 0.0), double(LEAVES - 1));
+    return static_cast<uint32_t>(r);
 }
 
 //! Monotone non-decreasing bucket of an order image x: the root feature,
 //! its conversion to double and every model step are monotone, leaf slopes
-//! are >= 0 and leaf bucket ranges are ordered.
+//! are >= 0 and leaf bucket ranges are ordered. The position a * x + b is
+//! truncated (clamped to the u32 range first) and clamped to the leaf's
+//! [lo, hi] in integers, which equals clamping it in doubles first: for
+//! integers lo <= hi, trunc(p) <= lo when p <= lo and >= hi when p >= hi.
 __device__ __forceinline__ uint32_t bucket_of(const rmi_model& m, uint64_t bits)
 {
     const double x = static_cast<double>(bits);
     const rmi_leaf& L = m.leaf[leaf_of(m, root_feature(m, bits))];
-    const double p = fma(L.a, x, L.b) * m.nb;
-    const double lo = L.lo, hi = L.hi;
-    return p <= lo ? L.lo : p >= hi ? L.hi : static_cast<uint32_t>(p);
+    const double p = fmin(fmax(fma(L.a, x, L.b), 0.0), 4294967295.0);
+    return min(max(static_cast<uint32_t>(p), L.lo), L.hi);
 }
 
 //! Onesweep digit functor: byte shift/8 of the key's bucket under model m.

@@ -278,8 +278,8 @@ __global__ __launch_bounds__(256) void leaf_fit_kernel(const uint64_t* __restric
         const uint32_t next =
             l + 1 < LEAVES ? static_cast<uint32_t>(uint64_t(nb) * hi / m) : nb - 1;
         rmi_leaf& L = models[c].leaf[l];
-        L.a = a;
-        L.b = b;
+        L.a = a * double(nb); // positions in buckets, not CDF fractions
+        L.b = b * double(nb);
         L.lo = min(blo, nb - 1);
         L.hi = max(L.lo, min(next, nb - 1));
     }
