@@ -433,9 +433,9 @@ __device__ __forceinline__ void digit_bases(uint32_t* s_hist, uint32_t* s_tmp)
     __syncthreads();
 }
 
-//! One stable LDS counting pass of a block over words w[] (row r of wave
-//! `wave` holds position wave * 64 * ITEMS + r * 64 + lane) by the nb-bit
-//! digit (w >> shift) & (2^nb - 1): wave-ballot ranks over nb bits,
+//! One stable LDS counting pass of a block over words w[] (row r < rows of
+//! wave `wave` holds position wave * 64 * rows + r * 64 + lane) by the
+//! nb-bit digit (w >> shift) & (2^nb - 1): wave-ballot ranks over nb bits,
 //! per-wave digit counters, an LDS reorder. Only positions below m are
 //! keys; rows of a wave that hold none are skipped, and the other positions
 //! past m take the largest digit, so they end behind every key (stability).
@@ -444,14 +444,14 @@ __device__ __forceinline__ void digit_bases(uint32_t* s_hist, uint32_t* s_tmp)
 //! (cleared while the reordered rows are read back).
 template <typename W, int MOVE, int BLOCK, int ITEMS>
 __device__ __forceinline__ void lds_pass(W (&w)[ITEMS], uint32_t (&v)[ITEMS], uint32_t shift,
-                                         uint32_t nb, uint32_t m, uint32_t* s_hist, W* s_w,
-                                         uint32_t* s_v, uint32_t* s_tmp)
+                                         uint32_t nb, uint32_t m, uint32_t rows,
+                                         uint32_t* s_hist, W* s_w, uint32_t* s_v,
+                                         uint32_t* s_tmp)
 {
     constexpr int NW = BLOCK / WAVE;
     const uint32_t tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
-    const uint32_t wave_base = wave * (WAVE * ITEMS);
-    const uint32_t live =
-        m > wave_base ? min(static_cast<uint32_t>(ITEMS), (m - wave_base + WAVE - 1) / WAVE) : 0u;
+    const uint32_t wave_base = wave * (WAVE * rows);
+    const uint32_t live = m > wave_base ? min(rows, (m - wave_base + WAVE - 1) / WAVE) : 0u;
     const uint32_t mask = (1u << nb) - 1u;
     uint32_t* my_hist = s_hist + wave * RADIX;
     uint32_t rank[ITEMS];
@@ -495,8 +495,9 @@ __device__ __forceinline__ void lds_pass(W (&w)[ITEMS], uint32_t (&v)[ITEMS], ui
 //! ceil(span / 8) passes of 8 bits (the last one narrower).
 template <typename W, int MOVE, int BLOCK, int ITEMS>
 __device__ __forceinline__ void lds_passes(W (&w)[ITEMS], uint32_t (&v)[ITEMS], uint32_t lo,
-                                           uint32_t span, uint32_t m, uint32_t* s_hist, W* s_w,
-                                           uint32_t* s_v, uint32_t* s_tmp)
+                                           uint32_t span, uint32_t m, uint32_t rows,
+                                           uint32_t* s_hist, W* s_w, uint32_t* s_v,
+                                           uint32_t* s_tmp)
 {
     const uint32_t passes = (span + 7) / 8;
     uint32_t shift = lo;
@@ -504,7 +505,7 @@ __device__ __forceinline__ void lds_passes(W (&w)[ITEMS], uint32_t (&v)[ITEMS], 
     for (uint32_t q = 0; q < passes; ++q)
     {
         const uint32_t nb = min(8u, span - 8 * q);
-        lds_pass<W, MOVE, BLOCK, ITEMS>(w, v, shift, nb, m, s_hist, s_w, s_v, s_tmp);
+        lds_pass<W, MOVE, BLOCK, ITEMS>(w, v, shift, nb, m, rows, s_hist, s_w, s_v, s_tmp);
         shift += nb;
     }
 }
@@ -535,9 +536,17 @@ __device__ __forceinline__ void lds_sort_segment(
 {
     constexpr bool WIDE = sizeof(K) == 8;
     constexpr uint32_t TILE = BLOCK * ITEMS;
+    constexpr uint32_t NW = BLOCK / WAVE;
     static_assert(TILE <= 65536, "positions are packed in 16 bits");
     const uint32_t tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
-    const uint32_t wave_base = wave * (WAVE * ITEMS);
+    // The segment is spread evenly over the waves, `rows` rows of 64 keys
+    // each (the waves wait on each other at every pass: a 4097-key segment
+    // in a 6144-key shape filled as 24 + 24 + 17 + 0 rows ran its passes at
+    // the pace of 24 rows; spread, 17 + 17 + 17 + 15). Rows r >= rows of a
+    // wave hold no positions.
+    const uint32_t rows = min(static_cast<uint32_t>(ITEMS),
+                              max(1u, (m + NW * WAVE - 1) / (NW * WAVE)));
+    const uint32_t wave_base = wave * (WAVE * rows);
 
     using U = typename std::conditional<WIDE, unsigned long long, unsigned int>::type;
     __shared__ U s_min, s_max;
@@ -546,8 +555,7 @@ __device__ __forceinline__ void lds_sort_segment(
         s_min = ~U(0);
         s_max = 0;
     }
-    const uint32_t live =
-        m > wave_base ? min(static_cast<uint32_t>(ITEMS), (m - wave_base + WAVE - 1) / WAVE) : 0u;
+    const uint32_t live = m > wave_base ? min(rows, (m - wave_base + WAVE - 1) / WAVE) : 0u;
     // Loads first, unconditionally (positions past m read key m - 1), so
     // they are all in flight together: no load waits on a lane or row test
     // (row tests here also cost the compiler registers).
@@ -569,7 +577,7 @@ __device__ __forceinline__ void lds_sort_segment(
     for (int r = 0; r < ITEMS; ++r)
     {
         k[r] = order_bits<K, KIND, DESC>(k[r]);
-        if (wave_base + r * WAVE + lane < m)
+        if (static_cast<uint32_t>(r) < rows && wave_base + r * WAVE + lane < m)
         {
             lo = min(lo, static_cast<U>(k[r]));
             hi = max(hi, static_cast<U>(k[r]));
@@ -610,8 +618,8 @@ __device__ __forceinline__ void lds_sort_segment(
         for (int r = 0; r < ITEMS; ++r)
             k[r] = (static_cast<uint64_t>(k[r] - kmin) << 16) | (wave_base + r * WAVE + lane);
         const bool wide = bits > 48;
-        lds_passes<uint64_t, 0, BLOCK, ITEMS>(k, unused, 16, min(bits, 48u), m, s_hist, s_w,
-                                              nullptr, s_tmp);
+        lds_passes<uint64_t, 0, BLOCK, ITEMS>(k, unused, 16, min(bits, 48u), m, rows, s_hist,
+                                              s_w, nullptr, s_tmp);
         if (wide)
         {
             // Positions past m fetch position 0 (in bounds, never used).
@@ -631,8 +639,8 @@ __device__ __forceinline__ void lds_sort_segment(
                     const uint64_t rel = order_bits<K, KIND, DESC>(hi16[r]) - kmin;
                     k[r] = ((rel >> 48) << 16) | (k[r] & 0xFFFFu);
                 }
-            lds_passes<uint64_t, 0, BLOCK, ITEMS>(k, unused, 16, bits - 48, m, s_hist, s_w,
-                                                  nullptr, s_tmp);
+            lds_passes<uint64_t, 0, BLOCK, ITEMS>(k, unused, 16, bits - 48, m, rows, s_hist,
+                                                  s_w, nullptr, s_tmp);
         }
         // Each value (and, for a wide range, each key) by its original
         // position; all are read before any is overwritten in place (the
@@ -653,7 +661,7 @@ __device__ __forceinline__ void lds_sort_segment(
         for (int r = 0; r < ITEMS; ++r)
         {
             const uint32_t pos = wave_base + r * WAVE + lane;
-            if (pos < m)
+            if (static_cast<uint32_t>(r) < rows && pos < m)
             {
                 keys[pos] = k[r];
                 vals[pos] = v[r];
@@ -667,12 +675,12 @@ __device__ __forceinline__ void lds_sort_segment(
     uint32_t* s_v = reinterpret_cast<uint32_t*>(s_w + TILE);
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r) k[r] -= kmin;
-    lds_passes<K, MOVE, BLOCK, ITEMS>(k, v, 0, bits, m, s_hist, s_w, s_v, s_tmp);
+    lds_passes<K, MOVE, BLOCK, ITEMS>(k, v, 0, bits, m, rows, s_hist, s_w, s_v, s_tmp);
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r)
     {
         const uint32_t pos = wave_base + r * WAVE + lane;
-        if (pos < m)
+        if (static_cast<uint32_t>(r) < rows && pos < m)
         {
             keys[pos] = from_order_bits<K, KIND, DESC>(static_cast<K>(k[r] + kmin));
             if constexpr (VALUES) vals[pos] = v[r];

@@ -30,8 +30,8 @@ __device__ __forceinline__ int bucket_of(uint64_t m)
 //! any kernel out of bounds.
 __global__ __launch_bounds__(256) void classify_kernel(
     const uint64_t* __restrict__ off, uint32_t nseg, uint64_t num_items, table t,
-    uint32_t large_cap, uint32_t* __restrict__ counts, uint4* __restrict__ entries,
-    uint32_t* __restrict__ large_list)
+    int use_mid, uint32_t large_cap, uint32_t* __restrict__ counts,
+    uint4* __restrict__ entries, uint32_t* __restrict__ large_list)
 {
     // The loop bound is uniform per block so that whole waves take part in
     // the per-class ballots: one atomic per class present in a wave (tens of
@@ -45,9 +45,14 @@ __global__ __launch_bounds__(256) void classify_kernel(
         {
             b = off[i];
             e = off[i + 1];
-            if (e > b + 1 && e <= num_items) c = t.cls[bucket_of(e - b)];
+            if (e > b + 1 && e <= num_items)
+            {
+                const int bk = bucket_of(e - b);
+                c = t.cls[bk];
+                if (use_mid && bk == MID_BUCKET && c < NCFG && e - b <= MID_CAP) c = MID;
+            }
         }
-        for (int k = 0; k <= LARGE; ++k)
+        for (int k = 0; k <= MID; ++k)
         {
             const uint64_t mask = __ballot(c == k);
             if (mask == 0) continue;
@@ -61,8 +66,8 @@ __global__ __launch_bounds__(256) void classify_kernel(
             {
                 if (pos < large_cap) large_list[pos] = i;
             }
-            else
-                entries[size_t(k) * nseg + pos] =
+            else // LDS classes; MID's list follows configuration NCFG - 1's
+                entries[size_t(k == MID ? NCFG : k) * nseg + pos] =
                     make_uint4(static_cast<uint32_t>(b), static_cast<uint32_t>(b >> 32),
                                static_cast<uint32_t>(e - b), 0u);
         }
@@ -505,20 +510,19 @@ uint32_t resident_grid(const context& ctx, F kernel, int block, std::size_t lds)
     return g;
 }
 
-//! Persistent grid of LDS configuration C (also opts the kernel in to more
-//! than 64 KiB of LDS). Computed once per instantiation, outside any
-//! stream capture.
-template <typename K, int KIND, bool DESC, bool VALUES, int C>
-uint32_t lds_class_grid(const context& ctx)
+//! Persistent grid of an LDS shape (also opts the kernel in to more than
+//! 64 KiB of LDS). Computed once per instantiation, outside any stream
+//! capture.
+template <typename K, int KIND, bool DESC, bool VALUES, int BLOCK, int ITEMS>
+uint32_t lds_shape_grid(const context& ctx)
 {
-    constexpr int BLOCK = CFG_BLOCK[C], ITEMS = CFG_ITEMS[C];
     static uint32_t grid = 0;
     static std::mutex m;
     std::lock_guard<std::mutex> lock(m);
     if (grid == 0)
     {
         auto kernel = lds_class_kernel<K, KIND, DESC, VALUES, BLOCK, ITEMS>;
-        const std::size_t lds = cfg_lds(C, sizeof(K), VALUES);
+        const std::size_t lds = shape_lds(BLOCK, std::size_t(BLOCK) * ITEMS, sizeof(K), VALUES);
         if (lds > 65536)
             CLC_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
                                         hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -528,17 +532,24 @@ uint32_t lds_class_grid(const context& ctx)
     return grid;
 }
 
+template <typename K, int KIND, bool DESC, bool VALUES, int BLOCK, int ITEMS>
+void launch_lds_shape(context& ctx, bool prepare, K* keys, uint32_t* vals,
+                      const uint4* entries, const uint32_t* count, uint32_t* queue)
+{
+    const uint32_t grid = lds_shape_grid<K, KIND, DESC, VALUES, BLOCK, ITEMS>(ctx);
+    if (prepare) return;
+    timed_launch timer(ctx, "segmented_lds_sort");
+    hipLaunchKernelGGL((lds_class_kernel<K, KIND, DESC, VALUES, BLOCK, ITEMS>), dim3(grid),
+                       dim3(BLOCK), shape_lds(BLOCK, std::size_t(BLOCK) * ITEMS, sizeof(K), VALUES),
+                       ctx.stream(), keys, vals, entries, count, queue);
+}
+
 template <typename K, int KIND, bool DESC, bool VALUES, int C>
 void launch_lds_class(context& ctx, bool prepare, K* keys, uint32_t* vals,
                       const uint4* entries, const uint32_t* count, uint32_t* queue)
 {
-    constexpr int BLOCK = CFG_BLOCK[C], ITEMS = CFG_ITEMS[C];
-    const uint32_t grid = lds_class_grid<K, KIND, DESC, VALUES, C>(ctx);
-    if (prepare) return;
-    timed_launch timer(ctx, "segmented_lds_sort");
-    hipLaunchKernelGGL((lds_class_kernel<K, KIND, DESC, VALUES, BLOCK, ITEMS>), dim3(grid),
-                       dim3(BLOCK), cfg_lds(C, sizeof(K), VALUES), ctx.stream(), keys,
-                       vals, entries, count, queue);
+    launch_lds_shape<K, KIND, DESC, VALUES, CFG_BLOCK[C], CFG_ITEMS[C]>(
+        ctx, prepare, keys, vals, entries, count, queue);
 }
 
 template <typename K, int KIND, bool DESC, bool VALUES>
@@ -603,6 +614,8 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
         else if (t.cls[b] >= 0 && t.cls[b] < NCFG) used[t.cls[b]] = true;
         else throw std::invalid_argument("segmented sort: bad size-class table");
     }
+    const bool use_mid = t.cls[MID_BUCKET] < NCFG
+        && mid_lds(sizeof(K), VALUES) <= ctx.facts().lds_per_block;
     const std::size_t m_min_large =
         first_large < NBUCKET ? (std::size_t(1) << (first_large - 1)) + 1 : 0;
     const std::size_t L_max =
@@ -615,7 +628,7 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
     const std::size_t o_stA = Lt.take(T_max * RADIX * 4);
     const std::size_t o_hist = Lt.take(L_max * P * RADIX * 4);
     const std::size_t zero_bytes = Lt.bytes;
-    const std::size_t o_entries = Lt.take(std::size_t(NCFG) * nseg * 16);
+    const std::size_t o_entries = Lt.take(std::size_t(NCFG + 1) * nseg * 16);
     const std::size_t o_llist = Lt.take(nseg * 4);
     const std::size_t o_tstart = Lt.take((L_max + 1) * 4);
     const std::size_t o_active = Lt.take(L_max * 4);
@@ -640,19 +653,23 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
             if (used[c])
                 launch_lds<K, KIND, DESC, VALUES>(ctx, true, c, nullptr, nullptr, nullptr,
                                                   nullptr, nullptr);
+        if (use_mid)
+            launch_lds_shape<K, KIND, DESC, VALUES, MID_BLOCK, MID_ITEMS>(
+                ctx, true, nullptr, nullptr, nullptr, nullptr, nullptr);
         return Lt.bytes;
     }
 
     temp_buffer tmp(ctx.resource(), Lt.bytes);
     char* base_ptr = tmp.as<char>();
     uint32_t* ctr = reinterpret_cast<uint32_t*>(base_ptr + o_ctr);
-    // counters: [0, NCFG] list counts, [8, 8 + NCFG) LDS-class queues,
+    // counters: [0, MID] list counts (LARGE's at LARGE, MID's at MID),
     // 16 histogram queue, 17 tiles in use, 18 large segments in use,
     // 19 copy-back queue, 20 active large segments, 21 their tiles,
     // [24, 24 + P) onesweep queues, [40, 40 + P) places that some large
-    // segment runs
+    // segment runs, [48, 48 + NCFG] LDS-class queues (MID's last)
+    static_assert(MID < 16 && 48 + NCFG + 1 <= 64, "counter layout");
     uint32_t* counts = ctr;
-    uint32_t* lds_q = ctr + 8;
+    uint32_t* lds_q = ctr + 48;
     uint32_t* hist_q = ctr + 16;
     uint32_t* total_tiles = ctr + 17;
     uint32_t* large_used = ctr + 18;
@@ -677,12 +694,16 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
         std::max<std::size_t>(1, std::min<std::size_t>((nseg + 255) / 256, 1024)));
     hipLaunchKernelGGL(classify_kernel, dim3(cgrid), dim3(256), 0, s, off,
                        static_cast<uint32_t>(nseg), static_cast<uint64_t>(num_items), t,
-                       static_cast<uint32_t>(L_max), counts, entries, llist);
+                       use_mid ? 1 : 0, static_cast<uint32_t>(L_max), counts, entries, llist);
     for (int c = 0; c < NCFG; ++c)
         if (used[c])
             launch_lds<K, KIND, DESC, VALUES>(ctx, false, c, keys, vals,
                                               entries + std::size_t(c) * nseg, counts + c,
                                               lds_q + c);
+    if (use_mid)
+        launch_lds_shape<K, KIND, DESC, VALUES, MID_BLOCK, MID_ITEMS>(
+            ctx, false, keys, vals, entries + std::size_t(NCFG) * nseg, counts + MID,
+            lds_q + NCFG);
     if (L_max > 0)
     {
         hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, s, off, llist,

@@ -41,6 +41,18 @@ constexpr int CFG_BLOCK[NCFG] = { 64, 64, 128, 256, 256, 512, 1024 };
 constexpr int CFG_ITEMS[NCFG] = { 4, 8, 8, 8, 16, 16, 16 };
 //! Class index of the large (onesweep) path.
 constexpr int LARGE = NCFG;
+//! An extra LDS shape for segments just above 4096 keys: bucket 13 holds
+//! 4097..8192, and a configuration that holds 8192 (512 x 16) runs a
+//! segment of 4097..6144 half empty (2^28 keys in 4097-key segments took
+//! 1.59x the time of 4096-key ones). When bucket 13's class is an LDS
+//! configuration, its segments of at most MID_CAP keys take this shape
+//! instead. Not a table entry (the tuned classes keep their meaning); class
+//! index MID internally.
+constexpr int MID = NCFG + 1;
+constexpr int MID_BLOCK = 256;
+constexpr int MID_ITEMS = 24;
+constexpr uint32_t MID_CAP = MID_BLOCK * MID_ITEMS;
+constexpr int MID_BUCKET = 13;
 //! Segment-size buckets: bucket b holds 2^(b-1) < m <= 2^b (m >= 2).
 constexpr int NBUCKET = 32;
 //! Onesweep tile of the large path.
@@ -66,12 +78,22 @@ inline std::size_t cfg_capacity(int c)
     return std::size_t(CFG_BLOCK[c]) * CFG_ITEMS[c];
 }
 
+//! Dynamic LDS bytes of an LDS shape (block, capacity) for key size kbytes.
+inline std::size_t shape_lds(int block, std::size_t tile, std::size_t kbytes, bool values)
+{
+    return std::size_t(block / 64) * 1024
+        + tile * radix::lds_segment_bytes_per_key(kbytes, values) + 64;
+}
+
 //! Dynamic LDS bytes of configuration c for key size kbytes.
 inline std::size_t cfg_lds(int c, std::size_t kbytes, bool values)
 {
-    const std::size_t tile = cfg_capacity(c);
-    return std::size_t(CFG_BLOCK[c] / 64) * 1024
-        + tile * radix::lds_segment_bytes_per_key(kbytes, values) + 64;
+    return shape_lds(CFG_BLOCK[c], cfg_capacity(c), kbytes, values);
+}
+
+inline std::size_t mid_lds(std::size_t kbytes, bool values)
+{
+    return shape_lds(MID_BLOCK, MID_CAP, kbytes, values);
 }
 
 //! Class per bucket; the autotuner fills it, defaults come from
