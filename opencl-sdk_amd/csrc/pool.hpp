@@ -61,7 +61,8 @@ public:
         std::size_t large_granularity = std::size_t(1) << 20;
         // The reference caches up to 1 GiB (MemoryResource.hpp:85); a 2^28
         // key-value sort here needs 2 GiB of alternate buffers, and an
-        // MI355X has 288 GB, so the default cap is 8 GiB.
+        // MI355X has 288 GB, so the default cap is 8 GiB (contexts raise
+        // it to an eighth of device memory).
         std::size_t max_cached_bytes = std::size_t(8) << 30;
     };
 
