@@ -25,12 +25,12 @@ workloads of the same path, each timed the same way:
              one captured clc_graph launch against one clc_sort call per
              segment; outputs compared bit for bit;
   primitives clc_scan (exclusive, "(a + b)" and the run-time compiled
-             "b + a + 0"), clc_reduce, clc_copy_if ("(x & 1) == 0") and
-             clc_histogram_even (256 bins) on 2^28 uint32: kernel time
-             (library hipEvent profiler), HBM roofline fraction at 8 / 4 /
-             4 + 4 x kept fraction / 4 algorithmic bytes per element,
-             outputs checked by difference / int64 sum / torch mask
-             selection / bincount;
+             "b + a + 0"), clc_reduce, clc_copy_if ("(x & 1) == 0"),
+             clc_transform ("x * 3u + 1u") and clc_histogram_even (256
+             bins) on 2^28 uint32: kernel time (library hipEvent profiler),
+             HBM roofline fraction at 8 / 4 / 4 + 4 x kept fraction / 8 / 4
+             algorithmic bytes per element, outputs checked by difference /
+             int64 sum / torch mask selection / torch arithmetic / bincount;
   batched    clc_learned_segmented_sort against clc_segmented_sort and
              against one clc_sort call per segment on 4096 float32 segments
              with log-uniform sizes 2^10..2^22 (about 2^31 pairs), each from
@@ -578,6 +578,14 @@ def primitives_leg(torch, clc_hip, ctx, steps, warmup):
     res["copy_if_even"] = dict(rate(ms["rtc_copy_if"], 4 + 4 * kept / n),
                                kept=kept, call_ms_incl_host_sync=round(wall, 4),
                                output_check=ok)
+    say("leg primitives transform")
+    out2 = torch.empty_like(x)
+    ms, wall = timed(lambda: ctx.transform(x, out2, "x * 3u + 1u", in_type=clc_hip.U32,
+                                           out_type=clc_hip.U32), ["rtc_transform"])
+    ok = bool(torch.equal(out2, x * 3 + 1))  # int32 arithmetic wraps as uint32
+    del out2
+    res["transform_u32"] = dict(rate(ms["rtc_transform"], 8), call_ms=round(wall, 4),
+                                output_check=ok)
     say("leg primitives histogram_even")
     ms, wall = timed(lambda: (hist.zero_(), ctx.histogram_even(
         x, hist, 0.0, 4294967296.0, type_code=clc_hip.U32)), ["histogram_even"])

@@ -8,7 +8,8 @@
 // Layout: a workgroup of BLOCK threads handles BLOCK * ITEMS consecutive
 // elements, item k of thread t at k * BLOCK + t, so every load and store
 // instruction of a wave covers 64 consecutive elements. All loads of a
-// thread are issued before its stores.
+// thread are issued before its stores; full tiles stream (non-temporal
+// loads and stores: 2^28 u32 "x * 3u + 1u" 0.375 -> 0.364 ms).
 #pragma once
 
 #include "device_common.hpp"
@@ -28,9 +29,11 @@ __device__ __forceinline__ void transform_tile(const TI* __restrict__ in,
     if (base + size_t(ITEMS - 1) * BLOCK < n)
     {
 #pragma unroll
-        for (int k = 0; k < ITEMS; ++k) x[k] = in[base + size_t(k) * BLOCK];
+        for (int k = 0; k < ITEMS; ++k)
+            x[k] = __builtin_nontemporal_load(in + base + size_t(k) * BLOCK);
 #pragma unroll
-        for (int k = 0; k < ITEMS; ++k) out[base + size_t(k) * BLOCK] = F::apply(x[k]);
+        for (int k = 0; k < ITEMS; ++k)
+            __builtin_nontemporal_store(F::apply(x[k]), out + base + size_t(k) * BLOCK);
     }
     else
     {
@@ -56,12 +59,12 @@ __device__ __forceinline__ void transform2_tile(const T1* __restrict__ in1,
 #pragma unroll
         for (int k = 0; k < ITEMS; ++k)
         {
-            x[k] = in1[base + size_t(k) * BLOCK];
-            y[k] = in2[base + size_t(k) * BLOCK];
+            x[k] = __builtin_nontemporal_load(in1 + base + size_t(k) * BLOCK);
+            y[k] = __builtin_nontemporal_load(in2 + base + size_t(k) * BLOCK);
         }
 #pragma unroll
         for (int k = 0; k < ITEMS; ++k)
-            out[base + size_t(k) * BLOCK] = F::apply(x[k], y[k]);
+            __builtin_nontemporal_store(F::apply(x[k], y[k]), out + base + size_t(k) * BLOCK);
     }
     else
     {

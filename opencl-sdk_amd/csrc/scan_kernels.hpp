@@ -501,7 +501,8 @@ __device__ __forceinline__ void select_tile(
     __syncthreads();
 
     T* dst = out + size_t(s_prefix) + wave_prefix;
-    for (uint32_t k = lane; k < kept; k += WAVE) dst[k] = s_stage[wave][k];
+    for (uint32_t k = lane; k < kept; k += WAVE)
+        __builtin_nontemporal_store(s_stage[wave][k], dst + k);
 }
 
 // ---- reduce --------------------------------------------------------------------
