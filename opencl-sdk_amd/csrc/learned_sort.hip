@@ -34,6 +34,7 @@
 #include "device_common.hpp"
 #include "learned_model.hpp"
 #include "learned_sort.hpp"
+#include "scan_kernels.hpp"
 #include "ops.hpp"
 #include "radix_tile.hpp"
 #include "segmented_sort.hpp"
@@ -365,10 +366,38 @@ __global__ __launch_bounds__(256) void id_histogram_kernel(const K* __restrict__
     for (uint32_t i = threadIdx.x; i < 2 * RADIX * C; i += 256) s_h[i] = 0;
     __syncthreads();
     const uint32_t copy = threadIdx.x % C;
-    // U independent loads per thread in flight before any inference.
-    constexpr uint32_t U = 4;
+    auto count = [&](K key) {
+        const uint32_t b = bucket_of(sm, static_cast<uint64_t>(order_bits<K, KIND, DESC>(key)));
+        atomicAdd(&s_h[(b & 0xFFu) * C + copy], 1u);
+        atomicAdd(&s_h[(RADIX + (b >> 8)) * C + copy], 1u);
+    };
     const uint32_t stride = gridDim.x * 256;
-    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += U * stride)
+    uint32_t start = 0;
+    if (reinterpret_cast<uintptr_t>(keys) % 16 == 0)
+    {
+        // Two 16-byte loads per thread in flight (streamed: read once here),
+        // clamped to the last vector; then the keys they hold.
+        constexpr uint32_t PER = 16 / sizeof(K);
+        const uint32_t nvec = n / PER;
+        const uint4* v = reinterpret_cast<const uint4*>(keys);
+        for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < nvec; i += 2 * stride)
+        {
+            const uint4 q0 = load_stream(v + i);
+            const uint4 q1 = load_stream(v + min(i + stride, nvec - 1));
+            const K* k0 = reinterpret_cast<const K*>(&q0);
+            const K* k1 = reinterpret_cast<const K*>(&q1);
+#pragma unroll
+            for (uint32_t j = 0; j < PER; ++j) count(k0[j]);
+            if (i + stride < nvec)
+#pragma unroll
+                for (uint32_t j = 0; j < PER; ++j) count(k1[j]);
+        }
+        start = nvec * PER;
+    }
+    // The rest (or an unaligned array): U independent loads per thread in
+    // flight before any inference.
+    constexpr uint32_t U = 4;
+    for (uint32_t i = start + blockIdx.x * 256 + threadIdx.x; i < n; i += U * stride)
     {
         K k[U];
 #pragma unroll
@@ -377,9 +406,7 @@ __global__ __launch_bounds__(256) void id_histogram_kernel(const K* __restrict__
         for (uint32_t u = 0; u < U; ++u)
         {
             if (i + u * stride >= n) break;
-            const uint32_t b = bucket_of(sm, static_cast<uint64_t>(order_bits<K, KIND, DESC>(k[u])));
-            atomicAdd(&s_h[(b & 0xFFu) * C + copy], 1u);
-            atomicAdd(&s_h[(RADIX + (b >> 8)) * C + copy], 1u);
+            count(k[u]);
         }
     }
     __syncthreads();
