@@ -2,8 +2,9 @@
 and the hipGraph executor), against the oracle's per-segment restatement
 of the reference sort. Bit-exact keys and value permutations.
 
-Covers: every size class (LDS configurations and the multi-workgroup
-onesweep path), class boundaries, empty and single-key segments, malformed
+Covers: every size class (LDS configurations, the 256 x 24 shape for
+4097..6144 keys and the multi-workgroup onesweep path), class boundaries,
+empty and single-key segments, malformed
 segments left untouched, every key type, both directions, keys-only, the
 lognormal batched configuration at full size (65,536 segments), tuned
 tables, and graph replays with eager work in between."""
@@ -44,7 +45,8 @@ def check(ctx, keys, vals, off, descending=False, threads=8):
 
 def boundary_offsets():
     sizes = [0, 1, 2, 3, 63, 64, 65, 255, 256, 257, 511, 512, 513, 1023, 1024, 1025,
-             2047, 2048, 2049, 4095, 4096, 4097, 5119, 5120, 5121, 8191, 8192, 8193,
+             2047, 2048, 2049, 4095, 4096, 4097, 5119, 5120, 5121, 6143, 6144, 6145,
+             8191, 8192, 8193,
              16383, 16384, 16385,
              20000, 40961, 100000, 262144, 300001]
     off = np.zeros(len(sizes) + 1, dtype=np.uint64)
