@@ -79,6 +79,14 @@ context::~context()
 {
     (void)hipSetDevice(device_);
     (void)hipStreamSynchronize(stream_);
+    for (hipStream_t& a : aux_streams_)
+        if (a)
+        {
+            (void)hipStreamSynchronize(a);
+            (void)hipStreamDestroy(a);
+        }
+    for (hipEvent_t& e : aux_events_)
+        if (e) (void)hipEventDestroy(e);
     programs_.reset(); // unloads run-time compiled modules
     if (pinned_) (void)hipHostFree(pinned_);
     resource_.reset();
@@ -203,6 +211,22 @@ void store_tuning(const context& ctx, const std::string& name,
         if (!f) return;
     }
     if (std::rename(tmp.c_str(), path.c_str()) != 0) std::remove(tmp.c_str());
+}
+
+hipStream_t context::aux_stream(int i)
+{
+    if (i < 0 || i >= AUX_STREAMS) throw std::out_of_range("clc: auxiliary stream index");
+    if (!aux_streams_[i])
+        CLC_HIP(hipStreamCreateWithFlags(&aux_streams_[i], hipStreamNonBlocking));
+    return aux_streams_[i];
+}
+
+hipEvent_t context::aux_event(int i)
+{
+    if (i < 0 || i >= AUX_EVENTS) throw std::out_of_range("clc: auxiliary event index");
+    if (!aux_events_[i])
+        CLC_HIP(hipEventCreateWithFlags(&aux_events_[i], hipEventDisableTiming));
+    return aux_events_[i];
 }
 
 rtc::program_cache& context::programs()

@@ -133,6 +133,14 @@ public:
     //! Run-time compiled operator programs (created on first use).
     rtc::program_cache& programs();
 
+    //! Auxiliary streams (default priority, non-blocking) for independent
+    //! branches of one operation, forked from and joined back to stream()
+    //! with the events below; created on first use, owned by the context.
+    static constexpr int AUX_STREAMS = 4;
+    static constexpr int AUX_EVENTS = 16;
+    hipStream_t aux_stream(int i);
+    hipEvent_t aux_event(int i);
+
     //! Temporarily redirects launches and scratch (used while a graph is
     //! captured on a private stream with graph-owned scratch); returns the
     //! previous values so the caller can restore them.
@@ -162,6 +170,8 @@ private:
     class profiler profiler_;
     std::map<int, std::vector<int>> tuned_;
     std::unique_ptr<rtc::program_cache> programs_;
+    hipStream_t aux_streams_[AUX_STREAMS] = {};
+    hipEvent_t aux_events_[AUX_EVENTS] = {};
 };
 
 //! Brackets one kernel launch with profiler events when profiling is on.

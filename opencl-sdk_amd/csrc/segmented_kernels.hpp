@@ -538,7 +538,9 @@ void launch_lds_shape(context& ctx, bool prepare, K* keys, uint32_t* vals,
 {
     const uint32_t grid = lds_shape_grid<K, KIND, DESC, VALUES, BLOCK, ITEMS>(ctx);
     if (prepare) return;
-    timed_launch timer(ctx, "segmented_lds_sort");
+    static const std::string name =
+        "segmented_lds_sort_" + std::to_string(BLOCK) + "x" + std::to_string(ITEMS);
+    timed_launch timer(ctx, name.c_str());
     hipLaunchKernelGGL((lds_class_kernel<K, KIND, DESC, VALUES, BLOCK, ITEMS>), dim3(grid),
                        dim3(BLOCK), shape_lds(BLOCK, std::size_t(BLOCK) * ITEMS, sizeof(K), VALUES),
                        ctx.stream(), keys, vals, entries, count, queue);
@@ -656,6 +658,9 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
         if (use_mid)
             launch_lds_shape<K, KIND, DESC, VALUES, MID_BLOCK, MID_ITEMS>(
                 ctx, true, nullptr, nullptr, nullptr, nullptr, nullptr);
+        // The branch streams and events exist before any capture.
+        for (int i = 0; i < context::AUX_STREAMS; ++i) ctx.aux_stream(i);
+        for (int i = 0; i < context::AUX_EVENTS; ++i) ctx.aux_event(i);
         return Lt.bytes;
     }
 
@@ -695,26 +700,18 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
     hipLaunchKernelGGL(classify_kernel, dim3(cgrid), dim3(256), 0, s, off,
                        static_cast<uint32_t>(nseg), static_cast<uint64_t>(num_items), t,
                        use_mid ? 1 : 0, static_cast<uint32_t>(L_max), counts, entries, llist);
-    for (int c = 0; c < NCFG; ++c)
-        if (used[c])
-            launch_lds<K, KIND, DESC, VALUES>(ctx, false, c, keys, vals,
-                                              entries + std::size_t(c) * nseg, counts + c,
-                                              lds_q + c);
-    if (use_mid)
-        launch_lds_shape<K, KIND, DESC, VALUES, MID_BLOCK, MID_ITEMS>(
-            ctx, false, keys, vals, entries + std::size_t(NCFG) * nseg, counts + MID,
-            lds_q + NCFG);
-    if (L_max > 0)
-    {
-        hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, s, off, llist,
+
+    // The large path (segments above every LDS capacity), on stream ls.
+    auto large_path = [&](hipStream_t ls) {
+        hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, ls, off, llist,
                            counts + LARGE, static_cast<uint32_t>(L_max),
                            static_cast<uint32_t>(T_max), TILE, tstart, large_used, total_tiles);
-        hipLaunchKernelGGL(hk, dim3(hgrid), dim3(256), 0, s, keys, off, llist, tstart,
+        hipLaunchKernelGGL(hk, dim3(hgrid), dim3(256), 0, ls, keys, off, llist, tstart,
                            large_used, total_tiles, hist, hist_q);
         const uint32_t sgrid = static_cast<uint32_t>(std::min<std::size_t>(L_max * P, 4096));
-        hipLaunchKernelGGL(large_digit_scan_kernel, dim3(sgrid), dim3(256), 0, s, hist,
+        hipLaunchKernelGGL(large_digit_scan_kernel, dim3(sgrid), dim3(256), 0, ls, hist,
                            dbase, large_used, uint32_t(P), off, llist, exec, any_exec);
-        hipLaunchKernelGGL(active_scan_kernel, dim3(1), dim3(1024), 0, s, off, llist,
+        hipLaunchKernelGGL(active_scan_kernel, dim3(1), dim3(1024), 0, ls, off, llist,
                            large_used, exec, uint32_t(P), static_cast<uint32_t>(T_max), TILE,
                            active, astart, act_used, act_tiles);
         K* kb[2] = { keys, reinterpret_cast<K*>(base_ptr + o_kalt) };
@@ -723,15 +720,61 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
         for (int p = 0; p < P; ++p)
         {
             timed_launch timer(ctx, "segmented_onesweep");
-            hipLaunchKernelGGL(ok, dim3(ogrid), dim3(LARGE_BLOCK), olds, s, kb[0], kb[1],
+            hipLaunchKernelGGL(ok, dim3(ogrid), dim3(LARGE_BLOCK), olds, ls, kb[0], kb[1],
                                vb[0], vb[1], off, llist, active, astart, act_used,
                                act_tiles, uint32_t(p), dbase, exec, any_exec, st[p % 2],
                                p + 1 < P ? st[(p + 1) % 2] : nullptr, pass_q + p);
         }
-        hipLaunchKernelGGL(ck, dim3(cpgrid), dim3(256), 0, s, kb[0], kb[1], vb[0], vb[1],
+        hipLaunchKernelGGL(ck, dim3(cpgrid), dim3(256), 0, ls, kb[0], kb[1], vb[0], vb[1],
                            off, llist, active, astart, act_used, act_tiles, exec,
                            uint32_t(P), copy_q);
-    }
+    };
+
+    // The large path and each LDS class sort disjoint segments: every one is
+    // a branch forked from stream s onto an auxiliary stream and joined back,
+    // so the classes' persistent kernels overlap (each drains its own queue,
+    // so a workgroup that starts after its queue is empty just exits)
+    // instead of running one after another with a drained GPU between them.
+    // Captured into a graph, the branches become parallel nodes.
+    hipEvent_t fork = ctx.aux_event(0);
+    CLC_HIP(hipEventRecord(fork, s));
+    int branch = 0;
+    struct stream_swap
+    {
+        context& c;
+        hipStream_t prev;
+        ~stream_swap() { c.swap_stream(prev); }
+    };
+    auto run_branch = [&](auto&& body) {
+        const hipStream_t as = ctx.aux_stream(branch % context::AUX_STREAMS);
+        CLC_HIP(hipStreamWaitEvent(as, fork, 0));
+        {
+            stream_swap guard{ ctx, ctx.swap_stream(as) };
+            body(as);
+        }
+        const hipEvent_t done = ctx.aux_event(1 + branch);
+        CLC_HIP(hipEventRecord(done, as));
+        CLC_HIP(hipStreamWaitEvent(s, done, 0));
+        ++branch;
+    };
+    static_assert(NCFG + 2 < context::AUX_EVENTS, "one event per branch and the fork");
+    if (L_max > 0)
+        run_branch([&](hipStream_t ls) {
+            large_path(ls);
+        });
+    for (int c = NCFG - 1; c >= 0; --c)
+        if (used[c])
+            run_branch([&](hipStream_t) {
+                launch_lds<K, KIND, DESC, VALUES>(ctx, false, c, keys, vals,
+                                                  entries + std::size_t(c) * nseg, counts + c,
+                                                  lds_q + c);
+            });
+    if (use_mid)
+        run_branch([&](hipStream_t) {
+            launch_lds_shape<K, KIND, DESC, VALUES, MID_BLOCK, MID_ITEMS>(
+                ctx, false, keys, vals, entries + std::size_t(NCFG) * nseg, counts + MID,
+                lds_q + NCFG);
+        });
     CLC_HIP(hipGetLastError());
     return Lt.bytes;
 }
