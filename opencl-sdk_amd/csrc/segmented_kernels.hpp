@@ -730,12 +730,15 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
                            uint32_t(P), copy_q);
     };
 
-    // The large path and each LDS class sort disjoint segments: every one is
-    // a branch forked from stream s onto an auxiliary stream and joined back,
-    // so the classes' persistent kernels overlap (each drains its own queue,
-    // so a workgroup that starts after its queue is empty just exits)
-    // instead of running one after another with a drained GPU between them.
-    // Captured into a graph, the branches become parallel nodes.
+    // The LDS classes sort disjoint segments: each is a branch forked from
+    // stream s onto an auxiliary stream and joined back, so the classes'
+    // persistent kernels overlap (each drains its own queue, so a workgroup
+    // that starts after its queue is empty just exits) instead of running
+    // one after another with a drained GPU between them. Captured into a
+    // graph, the branches become parallel nodes. The large path follows on
+    // s once they are done: overlapped with them as well, it measured slower
+    // on the lognormal batches (2.17 against 2.05 ms; its look-back chains
+    // stall behind other kernels' workgroups).
     hipEvent_t fork = ctx.aux_event(0);
     CLC_HIP(hipEventRecord(fork, s));
     int branch = 0;
@@ -758,10 +761,6 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
         ++branch;
     };
     static_assert(NCFG + 2 < context::AUX_EVENTS, "one event per branch and the fork");
-    if (L_max > 0)
-        run_branch([&](hipStream_t ls) {
-            large_path(ls);
-        });
     for (int c = NCFG - 1; c >= 0; --c)
         if (used[c])
             run_branch([&](hipStream_t) {
@@ -775,6 +774,7 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
                 ctx, false, keys, vals, entries + std::size_t(NCFG) * nseg, counts + MID,
                 lds_q + NCFG);
         });
+    if (L_max > 0) large_path(s);
     CLC_HIP(hipGetLastError());
     return Lt.bytes;
 }
