@@ -28,18 +28,29 @@ template <typename S> struct status_bits
 //! bits leaves 0 at the peers.
 __device__ __forceinline__ uint64_t match_digit(uint32_t d)
 {
+    // All eight ballots are taken before they are folded, so each lands in
+    // an SGPR pair of its own and the folds read ballots written a few
+    // instructions earlier: a ballot folded at once (through VCC) costs
+    // hazard wait states on gfx950 (s_nop in the code per bit: 512 -> 92 in
+    // the u32 pair pass).
+    uint32_t m[8];
+    uint64_t bal[8];
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+    {
+        m[b] = static_cast<uint32_t>(static_cast<int32_t>(d << (31 - b)) >> 31);
+        // Opaque to the optimiser, so the ballot compares m itself (one
+        // v_bfe_i32 + one v_cmp per bit, instead of re-deriving the bit with
+        // a second shift) and each half folds into one three-input op.
+        asm("" : "+v"(m[b]));
+        bal[b] = __ballot(m[b] != 0);
+    }
     uint32_t diff_lo = 0, diff_hi = 0;
 #pragma unroll
     for (int b = 0; b < 8; ++b)
     {
-        uint32_t m = static_cast<uint32_t>(static_cast<int32_t>(d << (31 - b)) >> 31);
-        // Opaque to the optimiser, so the ballot compares m itself (one
-        // v_bfe_i32 + one v_cmp per bit, instead of re-deriving the bit with
-        // a second shift) and each half folds into one three-input op.
-        asm("" : "+v"(m));
-        const uint64_t bal = __ballot(m != 0);
-        diff_lo |= static_cast<uint32_t>(bal) ^ m;
-        diff_hi |= static_cast<uint32_t>(bal >> 32) ^ m;
+        diff_lo |= static_cast<uint32_t>(bal[b]) ^ m[b];
+        diff_hi |= static_cast<uint32_t>(bal[b] >> 32) ^ m[b];
         // Keeps the accumulation a chain of diff | (bal ^ m) steps, each one
         // v_bitop3, rather than XORs re-associated into OR trees.
         asm("" : "+v"(diff_lo), "+v"(diff_hi));
