@@ -156,14 +156,18 @@ __device__ __forceinline__ void onesweep_load(const K* __restrict__ in, uint32_t
     constexpr uint32_t TILE = BLOCK * ITEMS;
     const uint32_t lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
     const uint32_t wave_base = tile * TILE + wave * (WAVE * ITEMS);
+    // One 64-bit base per thread and constant row offsets: indexing with
+    // the 32-bit idx of each row makes the compiler widen every row's
+    // address on its own (it cannot rule out a 32-bit wrap), two registers
+    // and a 64-bit add a row.
+    const K* p = in + (static_cast<size_t>(wave_base) + lane);
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r)
     {
-        const uint32_t idx = wave_base + r * WAVE + lane;
         if constexpr (FULL)
-            x[r] = in[idx];
+            x[r] = p[r * WAVE];
         else
-            x[r] = idx < n ? in[idx] : K(0);
+            x[r] = wave_base + r * WAVE + lane < n ? p[r * WAVE] : K(0);
     }
 }
 
