@@ -125,9 +125,14 @@ clc_status clc_finish(clc_context ctx);
 /* The context's hipStream_t. */
 clc_status clc_context_get_stream(clc_context ctx, void** hip_stream);
 
-/* Allocation statistics of the context's memory resource (the reference's
- * statistics_adaptor counters, MemoryResource.hpp:123-130, plus the pool's
- * cached bytes, MemoryResource.hpp:100-101). */
+/* Counters of the scratch pool behind the context (the role of the
+ * reference's statistics_adaptor counters, MemoryResource.hpp:123-130, and
+ * pool_resource::cached_bytes, :100-101). The pool belongs to the device and
+ * is shared by every context on it, so the figures cover all of them:
+ * device memory held now / at most / ever obtained, in bytes and blocks;
+ * the bytes parked for reuse; how many requests were served from the cache,
+ * how many of those took a block another stream had parked, and how many of
+ * those made the requesting stream wait on the device for the other one. */
 typedef struct clc_memory_stats
 {
     size_t current_bytes;
@@ -136,10 +141,14 @@ typedef struct clc_memory_stats
     size_t current_allocations;
     size_t total_allocations;
     size_t cached_bytes;
+    size_t reused;
+    size_t cross_stream;
+    size_t stream_waits;
 } clc_memory_stats;
 clc_status clc_context_memory_stats(clc_context ctx, clc_memory_stats* stats);
-/* Returns every cached block to the device (pool_resource::release,
- * MemoryResource.hpp:97-98). Synchronizes the context's stream first. */
+/* Synchronizes the context's stream, then returns every parked block of the
+ * device's pool to the device (waiting for the streams that parked them;
+ * the role of pool_resource::release, MemoryResource.hpp:97-98). */
 clc_status clc_context_release_cache(clc_context ctx);
 
 /* Counters of the context's cache of run-time compiled operator programs

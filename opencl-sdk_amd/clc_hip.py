@@ -47,7 +47,8 @@ class DeviceInfo(ctypes.Structure):
 class MemoryStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_size_t) for n in
                 ("current_bytes", "peak_bytes", "total_bytes",
-                 "current_allocations", "total_allocations", "cached_bytes")]
+                 "current_allocations", "total_allocations", "cached_bytes",
+                 "reused", "cross_stream", "stream_waits")]
 
 
 class ProgramStats(ctypes.Structure):

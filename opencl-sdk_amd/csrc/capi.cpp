@@ -28,12 +28,15 @@ struct clc_graph_s
     std::unique_ptr<clc::seg::graph> g;
 };
 
-// A block of the context's memory resource (the pool) and its size; the
-// resource is shared so the block can outlive the context, as in the
-// reference's clc_buffer_s (CApi.cpp:21-26).
+// A block of the device's scratch pool and its size. The pool is shared, so
+// the block can outlive the context, as a reference clc_buffer outlives its
+// context (CApi.cpp:21-26). The buffer is parked again on its context's
+// stream while that context lives (work on the buffer was queued there), and
+// with no owner stream after it: releasing a context waits for its stream.
 struct clc_buffer_s
 {
-    std::shared_ptr<clc::memory_resource> resource;
+    std::shared_ptr<clc::stream_pool> pool;
+    std::weak_ptr<const clc::stream_ref> home;
     void* data;
     std::size_t bytes;
 };
@@ -280,14 +283,16 @@ clc_status clc_context_memory_stats(clc_context ctx, clc_memory_stats* stats)
     if (ctx == nullptr || stats == nullptr)
         return fail(CLC_ERROR_INVALID_ARGUMENT, "clc: null argument");
     return guarded([&] {
-        clc::context& c = get(ctx);
-        const auto s = c.stats().statistics();
-        stats->current_bytes = s.current_bytes;
-        stats->peak_bytes = s.peak_bytes;
-        stats->total_bytes = s.total_bytes;
-        stats->current_allocations = s.current_allocations;
-        stats->total_allocations = s.total_allocations;
-        stats->cached_bytes = c.pool().cached_bytes();
+        const clc::stream_pool::figures f = get(ctx).pool()->tally();
+        stats->current_bytes = f.held_bytes;
+        stats->peak_bytes = f.held_peak;
+        stats->total_bytes = f.obtained_bytes;
+        stats->current_allocations = f.held_blocks;
+        stats->total_allocations = f.obtained_blocks;
+        stats->cached_bytes = f.parked_bytes;
+        stats->reused = f.reused;
+        stats->cross_stream = f.cross_stream;
+        stats->stream_waits = f.waits;
     });
 }
 
@@ -309,7 +314,7 @@ clc_status clc_context_release_cache(clc_context ctx)
     return guarded([&] {
         clc::context& c = get(ctx);
         CLC_HIP(hipStreamSynchronize(c.stream()));
-        c.pool().release();
+        c.pool()->trim();
     });
 }
 
@@ -396,8 +401,9 @@ clc_status clc_buffer_alloc(clc_context ctx, size_t bytes, clc_buffer* out)
     return guarded([&] {
         clc::context& c = get(ctx);
         c.activate();
-        std::unique_ptr<clc_buffer_s> b(new clc_buffer_s{ c.resource(), nullptr, bytes });
-        if (bytes != 0) b->data = b->resource->allocate(bytes);
+        std::unique_ptr<clc_buffer_s> b(
+            new clc_buffer_s{ c.pool(), c.stream_token(), nullptr, bytes });
+        if (bytes != 0) b->data = b->pool->take(bytes, c.stream());
         *out = b.release();
     });
 }
@@ -405,7 +411,11 @@ clc_status clc_buffer_alloc(clc_context ctx, size_t bytes, clc_buffer* out)
 void clc_buffer_release(clc_buffer buffer)
 {
     if (buffer == nullptr) return;
-    if (buffer->bytes != 0) buffer->resource->deallocate(buffer->data, buffer->bytes);
+    if (buffer->data != nullptr)
+    {
+        const std::shared_ptr<const clc::stream_ref> home = buffer->home.lock();
+        buffer->pool->give(buffer->data, buffer->bytes, home ? *home : clc::any_stream);
+    }
     delete buffer;
 }
 

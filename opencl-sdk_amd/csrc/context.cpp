@@ -6,6 +6,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <mutex>
 #include <unistd.h>
 
 namespace clc {
@@ -23,21 +24,88 @@ void hip_check(hipError_t e, const char* what)
     throw hip_error(e, msg);
 }
 
-void* device_resource::do_allocate(std::size_t bytes)
-{
-    int prev = 0;
-    CLC_HIP(hipGetDevice(&prev));
-    if (prev != device_) CLC_HIP(hipSetDevice(device_));
-    void* p = nullptr;
-    const hipError_t e = hipMalloc(&p, bytes == 0 ? 1 : bytes);
-    if (prev != device_) (void)hipSetDevice(prev);
-    hip_check(e, "hipMalloc");
-    return p;
-}
+namespace {
 
-void device_resource::do_deallocate(void* p, std::size_t) noexcept
+//! The pool's view of one HIP device.
+class hip_link final : public device_link {
+public:
+    explicit hip_link(int device) : device_(device) {}
+    ~hip_link() override
+    {
+        if (mark_) (void)hipEventDestroy(mark_);
+    }
+
+    void* obtain(std::size_t bytes) override
+    {
+        on_device here(device_);
+        void* p = nullptr;
+        hip_check(hipMalloc(&p, bytes == 0 ? 1 : bytes), "hipMalloc");
+        return p;
+    }
+    void surrender(void* p, std::size_t) noexcept override { (void)hipFree(p); }
+    bool idle(stream_ref s) override
+    {
+        const hipError_t e = hipStreamQuery(static_cast<hipStream_t>(s));
+        if (e == hipSuccess) return true;
+        (void)hipGetLastError(); // hipErrorNotReady is not an error here
+        return false;
+    }
+    void follow(stream_ref later, stream_ref earlier) override
+    {
+        // The pool serialises its calls, so one event serves every follow:
+        // a stream wait binds to the record that precedes it, and a later
+        // record of the same event does not move it.
+        if (!mark_)
+        {
+            on_device here(device_);
+            CLC_HIP(hipEventCreateWithFlags(&mark_, hipEventDisableTiming));
+        }
+        CLC_HIP(hipEventRecord(mark_, static_cast<hipStream_t>(earlier)));
+        CLC_HIP(hipStreamWaitEvent(static_cast<hipStream_t>(later), mark_, 0));
+    }
+    void drain(stream_ref s) noexcept override
+    {
+        (void)hipStreamSynchronize(static_cast<hipStream_t>(s));
+    }
+
+private:
+    //! Makes device current for a scope and restores the caller's device.
+    struct on_device
+    {
+        explicit on_device(int d)
+        {
+            (void)hipGetDevice(&prev);
+            if (prev != d) (void)hipSetDevice(d);
+            dev = d;
+        }
+        ~on_device()
+        {
+            if (prev != dev) (void)hipSetDevice(prev);
+        }
+        int prev = -1, dev = -1;
+    };
+    int device_;
+    hipEvent_t mark_ = nullptr;
+};
+
+} // namespace
+
+std::shared_ptr<stream_pool> device_pool(int device, std::size_t global_memory)
 {
-    (void)hipFree(p);
+    static std::mutex m;
+    static std::map<int, std::weak_ptr<stream_pool>> pools;
+    std::lock_guard<std::mutex> lock(m);
+    if (auto live = pools[device].lock()) return live;
+    // Cache up to an eighth of device memory (36 GB on a 288 GB MI355X): a
+    // 2^28 u64 key-value sort and a learned sort each keep 3+ GiB of
+    // alternate buffers, and a workload that alternates between them (or
+    // follows a u32 sort) must not return a block to hipFree and take it
+    // back from hipMalloc on every call (0.5 ms per 2^28-key learned sort).
+    stream_pool::limits l;
+    l.cache_cap = std::max(l.cache_cap, global_memory / 8);
+    auto fresh = std::make_shared<stream_pool>(std::make_shared<hip_link>(device), l);
+    pools[device] = fresh;
+    return fresh;
 }
 
 context::context(int device, hipStream_t stream, bool owns_stream,
@@ -60,28 +128,24 @@ context::context(int device, hipStream_t stream, bool owns_stream,
             ? 8
             : 1;
     if (owns_stream_) CLC_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-    stats_ = std::make_shared<statistics_adaptor>(
-        std::make_shared<device_resource>(device_));
-    // Cache up to an eighth of device memory (36 GB on a 288 GB MI355X; at
-    // least the pool's 8 GiB default): a 2^28 u64 key-value sort and a
-    // learned sort each keep 3+ GiB of alternate buffers, and with a smaller
-    // cap a workload that alternates between them (or follows a u32 sort)
-    // returned one block to hipFree and took it back from hipMalloc on
-    // every call, which cost 0.5 ms per 2^28-key learned sort.
-    pool_resource::options popts;
-    popts.max_cached_bytes = std::max(popts.max_cached_bytes, facts_.global_memory / 8);
-    pool_ = std::make_shared<pool_resource>(stats_, popts);
+    pool_ = device_pool(device_, facts_.global_memory);
     resource_ = pool_;
+    stream_token_ = std::make_shared<const stream_ref>(static_cast<stream_ref>(stream_));
     CLC_HIP(hipHostMalloc(&pinned_, 64, hipHostMallocDefault));
 }
 
 context::~context()
 {
     (void)hipSetDevice(device_);
+    stream_token_.reset();
     (void)hipStreamSynchronize(stream_);
+    // The pool outlives this context (other contexts and buffers share it):
+    // blocks parked by this context's streams must not name them any more.
+    pool_->retire(stream_);
     for (hipStream_t& a : aux_streams_)
         if (a)
         {
+            pool_->retire(a);
             (void)hipStreamSynchronize(a);
             (void)hipStreamDestroy(a);
         }
@@ -90,8 +154,7 @@ context::~context()
     programs_.reset(); // unloads run-time compiled modules
     if (pinned_) (void)hipHostFree(pinned_);
     resource_.reset();
-    pool_.reset(); // returns cached blocks
-    stats_.reset();
+    pool_.reset(); // the last holder returns the parked blocks
     if (owns_stream_) (void)hipStreamDestroy(stream_);
 }
 

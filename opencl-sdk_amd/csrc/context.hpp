@@ -1,6 +1,6 @@
 // Execution context and launch layer: one HIP device, one in-order stream,
-// a pool memory resource and the device facts the launch configurations
-// depend on. Plays the role of the reference's compute_context
+// the device's shared scratch pool and the device facts the launch
+// configurations depend on. Plays the role of the reference's compute_context
 // (lib/include/CL/Compute/Context.hpp:25-59, MemoryResource.cpp:169-197)
 // with the command queue replaced by a hipStream_t.
 #pragma once
@@ -13,7 +13,7 @@
 #include <string>
 #include <vector>
 
-#include "pool.hpp"
+#include "stream_pool.hpp"
 
 namespace clc {
 
@@ -40,19 +40,11 @@ void hip_check(hipError_t e, const char* what);
 
 #define CLC_HIP(call) ::clc::hip_check((call), #call)
 
-//! Upstream resource calling hipMalloc / hipFree for every block
-//! (buffer_resource, MemoryResource.hpp:52-61, MemoryResource.cpp:12-25).
-class device_resource final : public memory_resource {
-public:
-    explicit device_resource(int device) : device_(device) {}
-
-protected:
-    void* do_allocate(std::size_t bytes) override;
-    void do_deallocate(void* p, std::size_t bytes) noexcept override;
-
-private:
-    int device_;
-};
+//! The scratch pool of a device, shared by every context on it and by the
+//! buffers they hand out; created with the first context on the device and
+//! destroyed (its blocks returned) when the last holder lets go. Its cache
+//! cap is an eighth of the device's memory, at least 8 GiB.
+std::shared_ptr<stream_pool> device_pool(int device, std::size_t global_memory);
 
 //! Per-kernel launch timing with hipEvent pairs on the context's stream
 //! (the reference's samples time kernels with event profiling,
@@ -110,13 +102,14 @@ public:
 
     int device() const { return device_; }
     hipStream_t stream() const { return stream_; }
+    //! This context's stream for holders that may outlive the context
+    //! (buffers): expires when the context is destroyed.
+    std::weak_ptr<const stream_ref> stream_token() const { return stream_token_; }
     const device_facts& facts() const { return facts_; }
-    const std::shared_ptr<memory_resource>& resource() const
-    {
-        return resource_;
-    }
-    pool_resource& pool() { return *pool_; }
-    statistics_adaptor& stats() { return *stats_; }
+    //! Where this context's launches take scratch from: the device pool,
+    //! or a graph's own block while a graph is captured.
+    const std::shared_ptr<scratch_source>& resource() const { return resource_; }
+    const std::shared_ptr<stream_pool>& pool() const { return pool_; }
     const std::string& cache_dir() const { return cache_dir_; }
 
     //! Pinned host staging for small blocking reads (reduce results).
@@ -150,9 +143,9 @@ public:
         stream_ = s;
         return old;
     }
-    std::shared_ptr<memory_resource> swap_resource(std::shared_ptr<memory_resource> r)
+    std::shared_ptr<scratch_source> swap_resource(std::shared_ptr<scratch_source> r)
     {
-        std::shared_ptr<memory_resource> old = resource_;
+        std::shared_ptr<scratch_source> old = resource_;
         resource_ = std::move(r);
         return old;
     }
@@ -163,9 +156,9 @@ private:
     bool owns_stream_;
     std::string cache_dir_;
     device_facts facts_;
-    std::shared_ptr<statistics_adaptor> stats_;
-    std::shared_ptr<pool_resource> pool_;
-    std::shared_ptr<memory_resource> resource_;
+    std::shared_ptr<stream_pool> pool_;
+    std::shared_ptr<scratch_source> resource_;
+    std::shared_ptr<const stream_ref> stream_token_;
     void* pinned_ = nullptr;
     class profiler profiler_;
     std::map<int, std::vector<int>> tuned_;

@@ -583,7 +583,7 @@ void run(context& ctx, uint32_t* keys, uint32_t* vals, std::size_t n, const uint
     const std::size_t o_out = L.take((slots_max + 1) * 8);
     const std::size_t o_kalt = L.take(n * 4);
     const std::size_t o_valt = VALUES ? L.take(n * 4) : 0;
-    temp_buffer tmp(ctx.resource(), L.bytes);
+    temp_buffer tmp(ctx.resource(), L.bytes, ctx.stream());
     char* t = tmp.as<char>();
     uint32_t* totals = reinterpret_cast<uint32_t*>(t + o_totals);
     auto* stats = reinterpret_cast<unsigned long long*>(t + o_stats);

@@ -559,7 +559,7 @@ void run(context& ctx, K* keys, uint32_t* vals, uint32_t n, int key_type, stats*
     const std::size_t o_off = L.take((std::size_t(nb) + 1) * 8);
     const std::size_t o_kalt = L.take(std::size_t(n) * sizeof(K));
     const std::size_t o_valt = VALUES ? L.take(std::size_t(n) * 4) : 0;
-    temp_buffer tmp(ctx.resource(), L.bytes);
+    temp_buffer tmp(ctx.resource(), L.bytes, ctx.stream());
     char* t = tmp.as<char>();
     auto* quality = reinterpret_cast<unsigned long long*>(t + o_quality);
     auto* bstats = reinterpret_cast<unsigned long long*>(t + o_stats);

@@ -234,7 +234,7 @@ void sort_impl(context& ctx, K* keys, uint32_t* vals, uint32_t n, bool desc, int
     const std::size_t o_base = L.take(P * RADIX * 4);
     const std::size_t o_kalt = L.take(std::size_t(n) * sizeof(K));
     const std::size_t o_valt = values ? L.take(std::size_t(n) * 4) : 0;
-    temp_buffer tmp(ctx.resource(), L.bytes);
+    temp_buffer tmp(ctx.resource(), L.bytes, ctx.stream());
     char* t = tmp.as<char>();
     uint32_t* hist = reinterpret_cast<uint32_t*>(t + o_hist);
     uint32_t* ctr = reinterpret_cast<uint32_t*>(t + o_ctr);
@@ -347,8 +347,8 @@ int sort_tune(context& ctx, int key_type, bool with_values)
     ctx.activate();
     const std::size_t kb = key_bytes_of(key_type);
     constexpr uint32_t N = 1u << 26; // 512 MiB of pairs: past the 256 MB MALL
-    temp_buffer keys(ctx.resource(), std::size_t(N) * kb);
-    temp_buffer vals(ctx.resource(), with_values ? std::size_t(N) * 4 : 0);
+    temp_buffer keys(ctx.resource(), std::size_t(N) * kb, ctx.stream());
+    temp_buffer vals(ctx.resource(), with_values ? std::size_t(N) * 4 : 0, ctx.stream());
     hipEvent_t e0, e1;
     CLC_HIP(hipEventCreate(&e0));
     CLC_HIP(hipEventCreate(&e1));

@@ -90,7 +90,7 @@ void scan_typed(context& ctx, const void* in, void* out, size_t n,
                         && reinterpret_cast<uintptr_t>(out) % 16 == 0)
         ? 1 : 0;
     const size_t bytes = 256 + tiles * 16;
-    temp_buffer tmp(ctx.resource(), bytes);
+    temp_buffer tmp(ctx.resource(), bytes, ctx.stream());
     char* t = tmp.as<char>();
     detail::zero_fill(ctx.stream(), t, bytes);
     timed_launch tl(ctx, "scan");
@@ -114,7 +114,7 @@ void reduce_typed(context& ctx, const void* in, size_t n, const void* identity,
     const uint32_t groups = static_cast<uint32_t>(std::max<size_t>(
         1, std::min<size_t>((n + per_block - 1) / per_block,
                             size_t(ctx.facts().cu_count) * 8)));
-    temp_buffer partials(ctx.resource(), (groups + 1) * sizeof(T));
+    temp_buffer partials(ctx.resource(), (groups + 1) * sizeof(T), ctx.stream());
     T* part = partials.as<T>();
     const int vec_ok = reinterpret_cast<uintptr_t>(in) % 16 == 0 ? 1 : 0;
     {
@@ -257,7 +257,7 @@ void scan_expr(context& ctx, const void* in, void* out, std::size_t n, int type,
         ? 1 : 0;
     int ex = exclusive ? 1 : 0;
     const std::size_t bytes = 256 + tiles * 16;
-    temp_buffer tmp(ctx.resource(), bytes);
+    temp_buffer tmp(ctx.resource(), bytes, ctx.stream());
     char* t = tmp.as<char>();
     detail::zero_fill(ctx.stream(), t, bytes);
     void* status = t + 256;
@@ -289,7 +289,7 @@ void reduce_expr(context& ctx, const void* in, std::size_t n, int type,
     const uint32_t groups = static_cast<uint32_t>(std::max<std::size_t>(
         1, std::min<std::size_t>((n + per_block - 1) / per_block,
                                  std::size_t(ctx.facts().cu_count) * 8)));
-    temp_buffer partials(ctx.resource(), (groups + 1) * es);
+    temp_buffer partials(ctx.resource(), (groups + 1) * es, ctx.stream());
     char* part = partials.as<char>();
     unsigned char ini[8] = {};
     std::memcpy(ini, init, es);
@@ -335,7 +335,7 @@ std::size_t copy_if(context& ctx, const void* in, void* out, std::size_t n, int 
     const std::size_t tiles = (n + tile - 1) / tile;
     int vec_ok = reinterpret_cast<uintptr_t>(in) % 16 == 0 ? 1 : 0;
     const std::size_t bytes = 256 + tiles * 16;
-    temp_buffer tmp(ctx.resource(), bytes);
+    temp_buffer tmp(ctx.resource(), bytes, ctx.stream());
     char* t = tmp.as<char>();
     detail::zero_fill(ctx.stream(), t, bytes);
     void* status = t + 256;

@@ -143,9 +143,9 @@ table run_tuner(context& ctx, int key_type, bool values)
     table best = default_table(kb, values, limit);
     constexpr std::size_t TOTAL = std::size_t(1) << 22;
     constexpr int MAX_BUCKET = 22;
-    temp_buffer keys(ctx.resource(), TOTAL * kb);
-    temp_buffer vals(ctx.resource(), values ? TOTAL * 4 : 0);
-    temp_buffer offs(ctx.resource(), (TOTAL + 1) * 8);
+    temp_buffer keys(ctx.resource(), TOTAL * kb, ctx.stream());
+    temp_buffer vals(ctx.resource(), values ? TOTAL * 4 : 0, ctx.stream());
+    temp_buffer offs(ctx.resource(), (TOTAL + 1) * 8, ctx.stream());
     std::vector<uint64_t> host_off;
     hipEvent_t e0, e1;
     CLC_HIP(hipEventCreate(&e0));
@@ -209,21 +209,21 @@ table run_tuner(context& ctx, int key_type, bool values)
     return best;
 }
 
-//! Memory resource over one preallocated block: the scratch of a captured
-//! graph, which must outlive the capture and never be recycled.
-class fixed_resource final : public memory_resource {
+//! Scratch source over one block the graph holds: every launch recorded
+//! during the capture takes its scratch from it, and nothing is recycled
+//! while the graph can be replayed.
+class fixed_resource final : public scratch_source {
 public:
     fixed_resource(void* p, std::size_t bytes) : p_(p), bytes_(bytes) {}
 
-protected:
-    void* do_allocate(std::size_t bytes) override
+    void* take(std::size_t bytes, stream_ref) override
     {
         if (used_ || bytes > bytes_)
             throw std::logic_error("segmented sort graph: scratch already taken");
         used_ = true;
         return p_;
     }
-    void do_deallocate(void*, std::size_t) noexcept override {}
+    void give(void*, std::size_t, stream_ref) noexcept override {}
 
 private:
     void* p_;
@@ -239,7 +239,6 @@ graph::~graph()
 {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (g) (void)hipGraphDestroy(g);
-    if (scratch) (void)hipFree(scratch);
     if (capture_stream) (void)hipStreamDestroy(capture_stream);
 }
 
@@ -308,19 +307,20 @@ std::unique_ptr<seg::graph> segmented_sort_graph(context& ctx, void* keys, int k
     g->ctx = &ctx;
     const std::size_t bytes = fn(ctx, true, keys, values != nullptr, values, num_items,
                                  offsets, num_segments, descending, t);
-    CLC_HIP(hipMalloc(&g->scratch, bytes));
+    g->scratch.reset(new temp_buffer(ctx.resource(), bytes, ctx.stream()));
     CLC_HIP(hipStreamCreateWithFlags(&g->capture_stream, hipStreamNonBlocking));
     // Launch everything on a private stream under capture, drawing scratch
     // from the graph's own block and with event profiling off.
     const hipStream_t old_stream = ctx.swap_stream(g->capture_stream);
-    auto old_res = ctx.swap_resource(std::make_shared<seg::fixed_resource>(g->scratch, bytes));
+    auto old_res =
+        ctx.swap_resource(std::make_shared<seg::fixed_resource>(g->scratch->get(), bytes));
     const bool profiling = ctx.profiler().enabled();
     ctx.profiler().enable(false);
     struct restore
     {
         context& c;
         hipStream_t s;
-        std::shared_ptr<memory_resource> r;
+        std::shared_ptr<scratch_source> r;
         bool p;
         ~restore()
         {

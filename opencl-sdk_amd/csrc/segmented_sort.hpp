@@ -114,13 +114,15 @@ std::size_t sort_typed(context& ctx, bool prepare, void* keys, bool with_values,
                        std::size_t num_segments, bool descending, const table& t);
 
 //! A captured segmented sort: the whole launch sequence as one hipGraph,
-//! with scratch owned by the graph, replayed on the context's stream.
+//! replayed on the context's stream. Its scratch is one block of the device
+//! pool held for the graph's lifetime and parked again on the context's
+//! stream (where the replays were queued) when the graph is destroyed.
 struct graph
 {
     context* ctx = nullptr;
     hipGraph_t g = nullptr;
     hipGraphExec_t exec = nullptr;
-    void* scratch = nullptr;
+    std::unique_ptr<temp_buffer> scratch;
     hipStream_t capture_stream = nullptr;
     ~graph();
     void launch();

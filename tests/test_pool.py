@@ -1,6 +1,7 @@
-"""The temporary-storage allocator on the CPU: tests/cpp/pool_test.cpp built
-with g++ against opencl-sdk_amd/csrc/pool.cpp (no HIP involved), plus a
-ThreadSanitizer build of the same test for the pool's locking."""
+"""The per-GPU scratch pool on the CPU: tests/cpp/stream_pool_test.cpp built
+with g++ against opencl-sdk_amd/csrc/stream_pool.cpp and a fake device link
+(no HIP involved), plus ThreadSanitizer and AddressSanitizer builds of the
+same test for the pool's locking and its block bookkeeping."""
 import os
 import shutil
 import subprocess
@@ -8,12 +9,12 @@ import subprocess
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-SRC = [os.path.join(ROOT, "tests", "cpp", "pool_test.cpp"),
-       os.path.join(ROOT, "opencl-sdk_amd", "csrc", "pool.cpp")]
+SRC = [os.path.join(ROOT, "tests", "cpp", "stream_pool_test.cpp"),
+       os.path.join(ROOT, "opencl-sdk_amd", "csrc", "stream_pool.cpp")]
 
 
 def build_and_run(tmp_path, extra):
-    exe = str(tmp_path / "pool_test")
+    exe = str(tmp_path / "stream_pool_test")
     subprocess.run(["g++", "-std=c++17", "-O1", "-g", "-pthread", *extra, "-o", exe, *SRC],
                    check=True)
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
