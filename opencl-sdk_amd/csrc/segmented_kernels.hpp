@@ -25,9 +25,10 @@ __device__ __forceinline__ int bucket_of(uint64_t m)
 //! Files every segment of >= 2 keys under its class: an LDS class gets the
 //! entry (begin low, begin high, size, 0) in its list (stride nseg), the
 //! large class the segment's index. Segments that break the contract (end
-//! < begin, end > num_items) are left untouched, and the large class never
-//! takes more than large_cap segments, so malformed offsets cannot drive
-//! any kernel out of bounds.
+//! < begin, end > num_items, 2^30 keys or more) are left untouched, and the
+//! large class never takes more than large_cap segments, so malformed
+//! offsets cannot drive any kernel out of bounds. The size limit is the
+//! large path's: its look-back status words carry 30-bit digit counts.
 __global__ __launch_bounds__(256) void classify_kernel(
     const uint64_t* __restrict__ off, uint32_t nseg, uint64_t num_items, table t,
     int use_mid, uint32_t large_cap, uint32_t* __restrict__ counts,
@@ -45,7 +46,7 @@ __global__ __launch_bounds__(256) void classify_kernel(
         {
             b = off[i];
             e = off[i + 1];
-            if (e > b + 1 && e <= num_items)
+            if (e > b + 1 && e <= num_items && e - b < (uint64_t(1) << 30))
             {
                 const int bk = bucket_of(e - b);
                 c = t.cls[bk];
@@ -664,7 +665,7 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
         return Lt.bytes;
     }
 
-    temp_buffer tmp(ctx.resource(), Lt.bytes);
+    temp_buffer tmp(ctx.resource(), Lt.bytes, ctx.stream());
     char* base_ptr = tmp.as<char>();
     uint32_t* ctr = reinterpret_cast<uint32_t*>(base_ptr + o_ctr);
     // counters: [0, MID] list counts (LARGE's at LARGE, MID's at MID),

@@ -183,3 +183,77 @@ def test_errors(ctx):
     od = dev(np.array([0, 16], dtype=np.uint64))
     with pytest.raises(clc_hip.ClcError):
         ctx.learned_segmented_sort(keys, None, od, key_type=clc_hip.U64)
+
+
+def check_segmented_properties(orig, off_d, keys, vals, chunk=1 << 27):
+    """Size-independent check that (keys, vals) is the per-segment stable
+    ascending sort of the float32 keys orig, with vals the global index of
+    each key: within each segment the order images are non-decreasing;
+    every value lies in its own element's segment; the values form a
+    permutation; keys[i] == orig[vals[i]] bit for bit; and equal keys in a
+    segment keep index order. These facts fix each segment's stable sort
+    uniquely. Runs on the GPU in chunks to bound the temporaries."""
+    import torch
+    n = keys.numel()
+    kb = keys.view(torch.int32)
+    ob = orig.view(torch.int32)
+    # order image of float32 bits (the reference's KEY_TRANSFORM), as int64
+    def image(bits):
+        b = bits.to(torch.int64) & 0xFFFFFFFF
+        return torch.where(b >= 0x80000000, b ^ 0xFFFFFFFF, b | 0x80000000)
+    seen = torch.zeros(n, dtype=torch.bool, device="cuda")
+    ok = True
+    for a in range(0, n, chunk):
+        b = min(n, a + chunk + 1)  # one element of overlap for the pairwise tests
+        idx = torch.arange(a, b, dtype=torch.int64, device="cuda")
+        seg = torch.searchsorted(off_d, idx, right=True) - 1
+        lo, hi = off_d[seg], off_d[seg + 1]
+        v = vals[a:b].to(torch.int64) & 0xFFFFFFFF
+        ok = ok and bool(((v >= lo) & (v < hi)).all())
+        ok = ok and bool(torch.equal(ob[v], kb[a:b]))
+        img = image(kb[a:b])
+        same_seg = seg[1:] == seg[:-1]
+        ok = ok and bool((img[1:] >= img[:-1])[same_seg].all())
+        ties = same_seg & (img[1:] == img[:-1])
+        ok = ok and bool((v[1:][ties] > v[:-1][ties]).all())
+        seen[v] = True
+        del idx, seg, lo, hi, v, img, same_seg, ties
+        if not ok:
+            break
+    return ok and bool(seen.all())
+
+
+def test_c5_full_configuration(ctx):
+    """The batched learned configuration at full size, as the benchmark runs
+    it: 4096 float32 segments with sizes 2^U(10,22) (about 2.08e9 pairs),
+    generated on the GPU, u32 values = global index. Every segment is
+    checked on the GPU by the stable-sort properties; the three largest
+    segments are also compared with the oracle's per-segment sort."""
+    import torch
+    import bench
+    nseg = 4096
+    rng = np.random.default_rng(4096)
+    sizes = np.rint(2.0 ** rng.uniform(10.0, 22.0, nseg)).astype(np.int64)
+    off = np.zeros(nseg + 1, dtype=np.int64)
+    np.cumsum(sizes, out=off[1:])
+    total = int(off[-1])
+    say("learned batched full configuration", nseg, "segments,", total, "pairs")
+    g = torch.Generator(device="cuda")
+    g.manual_seed(4096)
+    keys = bench.c5_keys_gpu(torch, off, g)
+    orig = keys.clone()
+    vals = torch.arange(total, dtype=torch.int32, device="cuda")
+    off_d = torch.from_numpy(off).cuda()
+    st = ctx.learned_segmented_sort(keys, vals, off_d, key_type=clc_hip.F32)
+    torch.cuda.synchronize()
+    say("stats", st)
+    assert st["buckets"] > nseg
+    assert check_segmented_properties(orig, off_d, keys, vals)
+    for i in np.argsort(sizes)[-3:]:
+        b, e = int(off[i]), int(off[i + 1])
+        say("oracle check of segment", int(i), "size", e - b)
+        k0 = orig[b:e].cpu().numpy()
+        ek, ev = oracle.radix_sort(k0, np.arange(b, e, dtype=np.uint32), threads=8)
+        np.testing.assert_array_equal(keys[b:e].cpu().numpy().view(np.uint32),
+                                      ek.view(np.uint32))
+        np.testing.assert_array_equal(vals[b:e].cpu().numpy().view(np.uint32), ev)

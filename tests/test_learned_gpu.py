@@ -13,6 +13,7 @@ import pytest
 import clc_hip
 import oracle
 from gpu_helpers import TYPE_OF, dev, host, say
+from test_sort_gpu import check_stable_sort_properties
 
 pytestmark = pytest.mark.gpu
 
@@ -55,6 +56,10 @@ def test_distributions_u64_pairs(ctx, dist):
     say("learned", dist, n)
     st = check(ctx, DISTS[dist](n, 5))
     say("stats", st)
+    # The model is used (else this would compare the plain sort with the
+    # oracle again), with one bucket per ~4096 keys.
+    assert st["used_model"] == 1
+    assert st["num_buckets"] == 1024
     assert st["sample_size"] == n // 100
     assert 0.0 <= st["fallback_fraction"] <= 1.0
 
@@ -101,19 +106,32 @@ def test_degenerate_inputs(ctx, case):
     check(ctx, keys)
 
 
-def test_full_size_matches_plain_sort(ctx):
-    """2^28 u64 Gaussian-mixture keys with u32 values: the learned output
-    equals the plain radix sort's, bit for bit."""
+@pytest.mark.parametrize("dist", ["zipf_s1.2_2^40", "lognormal_20_2", "gauss_mix_8"])
+def test_full_size_2p28(ctx, dist):
+    """The learned configuration at full size: 2^28 u64 keys with u32 index
+    values from each skewed distribution, generated on the GPU as the
+    benchmark generates them. The output must satisfy the stable-sort
+    properties against the input (which fix it uniquely) and equal the plain
+    radix sort's bit for bit; the model must be used, with 65,536 buckets
+    (the two-byte bucket id, two id passes)."""
     import torch
+    import bench
     n = 1 << 28
-    say("learned vs plain at 2^28 (gaussian mixture u64)")
-    keys = torch.from_numpy(DISTS["gmm"](n, 11).view(np.int64)).cuda()
+    say("learned full size 2^28 u64 pairs", dist)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(1000 + list(bench.LEARNED_DISTS).index(dist))
+    keys = bench.LEARNED_DISTS[dist](torch, n, g)
+    orig = keys.clone()
     vals = torch.arange(n, dtype=torch.int32, device="cuda")
     k2, v2 = keys.clone(), vals.clone()
     st = ctx.learned_sort(keys, vals, key_type=clc_hip.U64)
     ctx.sort(k2, v2, key_type=clc_hip.U64)
     torch.cuda.synchronize()
+    say("stats", st)
+    assert st["used_model"] == 1
+    assert st["num_buckets"] == 65536
     same_keys = bool(torch.equal(keys, k2))
     same_vals = bool(torch.equal(vals, v2))
-    say("stats", st)
     assert same_keys and same_vals
+    del k2, v2
+    check_stable_sort_properties(orig, keys, vals)

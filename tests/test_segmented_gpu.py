@@ -131,13 +131,36 @@ def test_malformed_segments_untouched(ctx):
 
 
 def test_c3_full_size_lognormal_65536(ctx):
+    """The batched configuration at full size, through both entry points:
+    one direct clc_segmented_sort call and two replays of the captured
+    hipGraph (with eager work between them), each against the oracle."""
+    import torch
     say("segmented C3: 65536 lognormal segments (mu ln 512, sigma 1.5)")
     off = oracle.lognormal_segments(65536, np.log(512), 1.5, 1, 1 << 20, seed=3)
     n = int(off[-1])
     keys = oracle.pcg32_u32(42, 56, n)
     seg_of = np.repeat(np.arange(off.size - 1), np.diff(off).astype(np.int64))
     vals = (np.arange(n, dtype=np.uint64) - off[seg_of]).astype(np.uint32)
-    check(ctx, keys, vals, off, threads=16)
+    del seg_of
+    ek, ev = oracle.segmented_sort(keys, vals, off, threads=16)
+    k, v = run(ctx, keys, vals, off)
+    np.testing.assert_array_equal(k, ek)
+    np.testing.assert_array_equal(v, ev)
+    say("segmented C3: the same sort as one hipGraph launch")
+    kd, vd, od = dev(keys), dev(vals), dev(off)
+    g = ctx.segmented_sort_graph(kd, vd, od, key_type=clc_hip.U32)
+    try:
+        for rep in range(2):
+            if rep:
+                kd.copy_(dev(keys))
+                vd.copy_(dev(vals))
+                ctx.sort(dev(keys[:5000]), None, key_type=clc_hip.U32)
+            g.launch()
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(host(kd, np.uint32), ek)
+            np.testing.assert_array_equal(host(vd, np.uint32), ev)
+    finally:
+        g.close()
 
 
 def test_tuned_table_and_rerun(ctx):

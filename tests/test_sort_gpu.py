@@ -62,6 +62,17 @@ def test_reference_sizes_large(ctx, n):
     np.testing.assert_array_equal(v, ev)
 
 
+def test_c1_keys_only_2p20_pcg32(ctx):
+    """The reference test's largest size (main.cpp:54) as the CPU-runnable
+    configuration states it: 2^20 raw pcg32 outputs (srandom(42, 54)),
+    keys only, ascending."""
+    n = 1 << 20
+    say("sort C1 keys-only 2^20 pcg32")
+    keys = oracle.pcg32_u32(42, 54, n)
+    k, _ = gpu_sort(ctx, keys)
+    np.testing.assert_array_equal(k, oracle.radix_sort(keys, threads=8)[0])
+
+
 def test_reference_stability_vector(ctx, golden):
     say("sort stability 50000 keys in [0,15]")
     keys = golden["stable_keys_u8"].astype(np.uint32)
