@@ -235,6 +235,21 @@ clc_status clc_scan(clc_context ctx, const void* in, void* out, size_t n,
                     clc_type type, const char* op, const char* identity,
                     int exclusive);
 
+/* clc_scan that allocates nothing, for callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). No counterpart in the reference. Same arguments and
+ * results as clc_scan, but only the canonical ops:: operators and
+ * identities (an expression would be compiled, which allocates:
+ * CLC_ERROR_UNSUPPORTED) and no profiler events. Its look-back scratch is
+ * a static device array of 4 MiB + 256 bytes per GPU, held until the
+ * process ends, which covers every n up to 2^31-1 for every element type
+ * (larger needs are CLC_ERROR_INVALID_ARGUMENT). All calls on one GPU share
+ * that array: calls that overlap on one GPU (other streams, other threads,
+ * other contexts) race and must be serialised by the caller. */
+clc_status clc_scan_noalloc(clc_context ctx, const void* in, void* out, size_t n,
+                            clc_type type, const char* op, const char* identity,
+                            int exclusive);
+
 /* out[i] = value for i < n (value: one host element of type).
  * (reference clc_fill, compute_c.h:142-143; Algorithm.cpp:331-343) */
 clc_status clc_fill(clc_context ctx, void* out, size_t n, clc_type type, const void* value);

@@ -111,6 +111,7 @@ SIGNATURES = {
     "clc_context_reset_timings": (_i, [_vp]),
     "clc_reduce": (_i, [_vp, _vp, _sz, _i, _cp, _cp, _vp, _vp]),
     "clc_scan": (_i, [_vp, _vp, _vp, _sz, _i, _cp, _cp, _i]),
+    "clc_scan_noalloc": (_i, [_vp, _vp, _vp, _sz, _i, _cp, _cp, _i]),
     "clc_sort": (_i, [_vp, _vp, _i, _vp, _sz, _i]),
     "clc_histogram_even": (_i, [_vp, _vp, _sz, _i, _vp, _sz, ctypes.c_double,
                                 ctypes.c_double]),

@@ -393,6 +393,23 @@ clc_status clc_scan(clc_context ctx, const void* in, void* out, size_t n,
     });
 }
 
+clc_status clc_scan_noalloc(clc_context ctx, const void* in, void* out, size_t n,
+                            clc_type type, const char* op, const char* identity,
+                            int exclusive)
+{
+    return guarded([&] {
+        require(ctx != nullptr && op != nullptr && identity != nullptr,
+                "clc: null argument");
+        clc::op_code code;
+        unsigned char id[8];
+        if (!canonical(type, op, identity, code, id))
+            throw clc::unsupported_error(
+                "clc_scan_noalloc: only the canonical operators and identities; "
+                "compiling an expression allocates");
+        clc::detail::scan_noalloc(get(ctx), in, out, n, type, code, id, exclusive != 0);
+    });
+}
+
 clc_status clc_buffer_alloc(clc_context ctx, size_t bytes, clc_buffer* out)
 {
     if (out == nullptr) return fail(CLC_ERROR_INVALID_ARGUMENT, "clc: null argument");

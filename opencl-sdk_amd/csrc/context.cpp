@@ -132,6 +132,9 @@ context::context(int device, hipStream_t stream, bool owns_stream,
     resource_ = pool_;
     stream_token_ = std::make_shared<const stream_ref>(static_cast<stream_ref>(stream_));
     CLC_HIP(hipHostMalloc(&pinned_, 64, hipHostMallocDefault));
+    // The allocation-free scan's array address on this device. A failure
+    // here is reported by that entry point, which retries the lookup.
+    (void)detail::scan_noalloc_setup(device_);
 }
 
 context::~context()

@@ -216,6 +216,14 @@ void radix_sort(context& ctx, void* keys, int key_type, void* values,
                 std::size_t n, bool descending);
 void scan(context& ctx, const void* in, void* out, std::size_t n, int type,
           int op, const void* identity, bool exclusive);
+//! scan with its look-back scratch in a static per-GPU device array instead
+//! of the pool: no device allocation on any call. Canonical operators only.
+//! Calls that overlap on one GPU race for the array.
+void scan_noalloc(context& ctx, const void* in, void* out, std::size_t n, int type,
+                  int op, const void* identity, bool exclusive);
+//! Looks up the address of scan_noalloc's array on device (which must be
+//! current); once per device, at context creation.
+hipError_t scan_noalloc_setup(int device);
 void reduce(context& ctx, const void* in, std::size_t n, int type, int op,
             const void* identity, const void* init, void* result);
 //! scan / reduce with an operator and identity given as OpenCL C

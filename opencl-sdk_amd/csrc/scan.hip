@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 namespace clc {
 namespace {
@@ -76,9 +77,19 @@ __global__ __launch_bounds__(BLOCK) void reduce_kernel(
 
 // ---- host side -------------------------------------------------------------------
 
+//! Scratch bytes of a scan of n elements of T: the tile counter (256 B)
+//! and one 16-byte look-back status slot per tile.
+template <typename T> size_t scan_scratch_bytes(size_t n)
+{
+    constexpr size_t TILE = size_t(SCAN_BLOCK) * scan_items<T>();
+    return 256 + (n + TILE - 1) / TILE * 16;
+}
+
+//! Clears the scratch t (scan_scratch_bytes<T>(n) bytes) with a kernel and
+//! launches the scan on ctx's stream.
 template <typename T, int OP>
-void scan_typed(context& ctx, const void* in, void* out, size_t n,
-                const void* identity, bool exclusive)
+void scan_launch(context& ctx, const void* in, void* out, size_t n,
+                 const void* identity, bool exclusive, char* t)
 {
     constexpr int BLOCK = SCAN_BLOCK;
     constexpr int ITEMS = scan_items<T>();
@@ -89,17 +100,23 @@ void scan_typed(context& ctx, const void* in, void* out, size_t n,
     const int vec_ok = (reinterpret_cast<uintptr_t>(in) % 16 == 0
                         && reinterpret_cast<uintptr_t>(out) % 16 == 0)
         ? 1 : 0;
-    const size_t bytes = 256 + tiles * 16;
-    temp_buffer tmp(ctx.resource(), bytes, ctx.stream());
-    char* t = tmp.as<char>();
-    detail::zero_fill(ctx.stream(), t, bytes);
-    timed_launch tl(ctx, "scan");
+    detail::zero_fill(ctx.stream(), t, scan_scratch_bytes<T>(n));
     hipLaunchKernelGGL((scan_kernel<T, OP, BLOCK, ITEMS>), dim3(tiles), dim3(BLOCK),
                        0, ctx.stream(), static_cast<const T*>(in),
                        static_cast<T*>(out), n, id, exclusive ? 1 : 0, vec_ok,
                        reinterpret_cast<uint64_t*>(t + 256),
                        reinterpret_cast<uint32_t*>(t));
     CLC_HIP(hipGetLastError());
+}
+
+template <typename T, int OP>
+void scan_typed(context& ctx, const void* in, void* out, size_t n,
+                const void* identity, bool exclusive)
+{
+    const size_t bytes = scan_scratch_bytes<T>(n);
+    temp_buffer tmp(ctx.resource(), bytes, ctx.stream());
+    timed_launch tl(ctx, "scan");
+    scan_launch<T, OP>(ctx, in, out, n, identity, exclusive, tmp.as<char>());
 }
 
 template <typename T, int OP>
@@ -197,6 +214,18 @@ template <typename T, int OP> struct scan_fn
     }
 };
 
+template <typename T, int OP> struct scan_fixed_fn
+{
+    static void run(context& ctx, const void* in, void* out, size_t n,
+                    const void* id, bool ex, char* scratch_ptr, size_t capacity)
+    {
+        if (scan_scratch_bytes<T>(n) > capacity)
+            throw std::length_error("clc_scan_noalloc: the call needs more than the "
+                                    "fixed scratch holds");
+        scan_launch<T, OP>(ctx, in, out, n, id, ex, scratch_ptr);
+    }
+};
+
 template <typename T, int OP> struct reduce_fn
 {
     static void run(context& ctx, const void* in, size_t n, const void* id,
@@ -206,9 +235,68 @@ template <typename T, int OP> struct reduce_fn
     }
 };
 
+// ---- scan without allocation ------------------------------------------------
+// Scratch of clc_scan_noalloc in a static device array, one copy per GPU
+// (the loader places it with the code object; each GPU keeps its copy until
+// the process ends). The largest call the C ABI takes, 2^31-1 elements of 8
+// bytes at 8192 a tile, needs 256 + 262,144 x 16 bytes; smaller elements
+// take fewer tiles, so every legal size fits. Host code reaches the array
+// only through the address hipGetSymbolAddress returns on each device.
+constexpr size_t NOALLOC_SCRATCH_BYTES = 256 + (size_t(1) << 22);
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_DEVICES = 64;
+std::mutex noalloc_mutex;
+char* noalloc_scratch[NOALLOC_DEVICES] = {};
+
 } // namespace
 
 namespace detail {
+
+hipError_t scan_noalloc_setup(int device)
+{
+    if (device < 0 || device >= NOALLOC_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(noalloc_mutex);
+    if (noalloc_scratch[device] != nullptr) return hipSuccess;
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device) return hipErrorInvalidDevice; // the caller makes it current
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess)
+    {
+        (void)hipGetLastError();
+        return e;
+    }
+    noalloc_scratch[device] = static_cast<char*>(p);
+    return hipSuccess;
+}
+
+void scan_noalloc(context& ctx, const void* in, void* out, std::size_t n, int type,
+                  int op, const void* identity, bool exclusive)
+{
+    if (n == 0) return;
+    check_size(n);
+    if (in == nullptr || out == nullptr)
+        throw std::invalid_argument("clc_scan_noalloc: null buffer");
+    ctx.activate();
+    char* p = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(noalloc_mutex);
+        if (ctx.device() >= 0 && ctx.device() < NOALLOC_DEVICES)
+            p = noalloc_scratch[ctx.device()];
+    }
+    if (p == nullptr)
+    {
+        hip_check(scan_noalloc_setup(ctx.device()),
+                  "clc_scan_noalloc: hipGetSymbolAddress of the scratch array");
+        std::lock_guard<std::mutex> lock(noalloc_mutex);
+        p = noalloc_scratch[ctx.device()];
+    }
+    by_type<scan_fixed_fn>(type, op, ctx, in, out, n, identity, exclusive, p,
+                           NOALLOC_SCRATCH_BYTES);
+}
 
 void scan(context& ctx, const void* in, void* out, std::size_t n, int type,
           int op, const void* identity, bool exclusive)
