@@ -227,3 +227,16 @@ def test_buffer_handles_without_device():
     L.clc_buffer_release(None)
     assert L.clc_buffer_read(None, None, 0, 0, None) == clc_hip.ERROR_INVALID_ARGUMENT
     assert L.clc_buffer_copy(None, None, 0, None, 0, 0) == clc_hip.ERROR_INVALID_ARGUMENT
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+def test_cxx_header_compiles_and_links(tmp_path):
+    """include/compute.hpp (the C++ algorithm API over the C ABI) compiles
+    with warnings as errors, and the C++ GPU test built on it links against
+    the library (running it needs a GPU)."""
+    from test_compute_api_gpu import compile_cmd
+    exe = str(tmp_path / "compute_api_test")
+    r = subprocess.run(compile_cmd(os.path.join(ROOT, "tests", "cpp", "compute_api_test.cpp"),
+                                   exe), capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert os.path.exists(exe)
