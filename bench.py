@@ -1,11 +1,12 @@
 #!/usr/bin/env python3
 """Benchmark of the hot path on one MI355X, through the C ABI.
 
-Main line: stable key-value radix sort (clc_sort) of 2^28 uint32 keys
-uniform over 32 bits with uint32 values 0..n-1, ascending.
+Main line: stable key-value radix sort (clc_sort) of 2^28 uint32 keys, the
+raw outputs of the reference's pcg32 generator seeded srandom(42, 54), with
+uint32 values 0..n-1, ascending.
 
     python bench.py [--steps K] [--warmup W] [--log2n 28] [--key u32|u64]
-                    [--legs u64,learned,segmented,batched,primitives | none]
+                    [--legs c1,u64,learned,segmented,batched,primitives | none]
                     [--no-cpu]
 
 A step is one clc_sort call on inputs already resident in HBM. Before each
@@ -15,7 +16,11 @@ Prints one JSON line: metric, value in GKeys/s, the roofline of the dominant
 kernel (onesweep pass), the CPU baseline, and under "legs" the other
 workloads of the same path, each timed the same way:
 
-  u64        2^27 uint64 keys uniform over [0, 2^48) + uint32 values;
+  c1         2^20 pcg32 keys sorted keys-only and an exclusive scan of
+             2^20 values pcg32 & 0xFF (Infinity-Cache resident: rates are
+             reported, not judged on the HBM roofline);
+  u64        2^27 uint64 keys uniform over [0, 2^48) (two pcg32 draws,
+             (hi << 32 | lo) masked) + uint32 values;
   learned    clc_learned_sort against clc_sort on 2^28 uint64 keys + uint32
              values from three skewed distributions (Zipf s=1.2 over
              [1, 2^40], round(lognormal(20, 2)), 8 Gaussians with means in
@@ -70,7 +75,12 @@ def cpu_baseline(log2n: int, reps: int) -> dict:
     rng = np.random.default_rng(12345)
     keys = rng.integers(0, 1 << 32, size=n, dtype=np.uint32)
     vals = np.arange(n, dtype=np.uint32)
-    threads = max(1, min(os.cpu_count() or 1, 16))
+    # The host cores this process may run on, capped by the thread budget
+    # the environment grants it (OMP_NUM_THREADS: a one-GPU box's CPU share).
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (
+        os.cpu_count() or 1)
+    budget = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = max(1, min(avail, budget) if budget > 0 else avail)
     times = []
     for _ in range(reps):
         t = time.perf_counter()
@@ -78,10 +88,44 @@ def cpu_baseline(log2n: int, reps: int) -> dict:
         times.append(time.perf_counter() - t)
     best = min(times)
     return {"value": round(n / best / 1e9, 4), "unit": "GKeys/s", "cores": threads,
-            "kind": "port",
+            "kind": "port", "host_cpus": os.cpu_count(), "cpus_available": avail,
             "sample": f"2^{log2n} uniform u32 keys + u32 index values, best of {reps} "
                       f"runs of the oracle's 4-bit LSD restatement "
-                      f"({threads} threads, {best:.2f} s each)"}
+                      f"({threads} threads of {os.cpu_count()} host CPUs, "
+                      f"{best:.2f} s each)"}
+
+
+PCG_MULT = 6364136223846793005  # pcg32's LCG multiplier (O'Neill, pcg-random.org)
+
+
+def pcg32_stream(initstate: int, initseq: int, n: int):
+    """The first n outputs of pcg32_random_r after pcg32_srandom_r(initstate,
+    initseq) (the reference's generator, lib/src/SDK/Random.c:13-50), as a
+    numpy uint32 array. The states are produced by doubling: a block of
+    states advanced by its own length is one multiply-add with the composed
+    LCG step (A, C) -> (A^2, A*C + C); then the output function
+    (xorshift-high, random rotation) is applied to every state at once."""
+    import numpy as np
+    mask = (1 << 64) - 1
+    inc = ((initseq << 1) | 1) & mask
+    state = (inc + initstate) & mask          # srandom: one step from 0, += initstate
+    state = (state * PCG_MULT + inc) & mask   # ... and one more step
+    states = np.empty(n, dtype=np.uint64)
+    if n == 0:
+        return states.astype(np.uint32)
+    states[0] = state
+    a, c, have = PCG_MULT, inc, 1
+    with np.errstate(over="ignore"):
+        while have < n:
+            take = min(have, n - have)
+            np.multiply(states[:take], np.uint64(a), out=states[have:have + take])
+            states[have:have + take] += np.uint64(c)
+            have += take
+            a, c = (a * a) & mask, (c * a + c) & mask
+        x = (((states >> np.uint64(18)) ^ states) >> np.uint64(27)).astype(np.uint32)
+        rot = (states >> np.uint64(59)).astype(np.uint32)
+        del states
+        return (x >> rot) | (x << ((np.uint32(32) - rot) & np.uint32(31)))
 
 
 def event_pairs(torch, k):
@@ -108,13 +152,18 @@ def plain_sort(torch, clc_hip, ctx, wide, log2n, steps, warmup, traffic_dir=None
     vbytes = 4 if values else 0
     passes = 8 if wide else 4
     executed = 6 if wide else 4  # u64 keys below 2^48: the top two bytes are constant
-    g = torch.Generator(device="cuda")
-    g.manual_seed(42)
+    import numpy as np
+    # Seeded pcg32 streams: u32 keys are raw outputs of srandom(42, 54); u64
+    # keys are two draws of srandom(42, 56), (hi << 32 | lo) & (2^48 - 1).
     if wide:
-        keys0 = torch.randint(0, 1 << 48, (n,), dtype=torch.int64, device="cuda", generator=g)
+        raw = pcg32_stream(42, 56, 2 * n).astype(np.uint64)
+        host_keys = ((raw[0::2] << np.uint64(32)) | raw[1::2]) & np.uint64((1 << 48) - 1)
+        del raw
+        keys0 = torch.from_numpy(host_keys.view(np.int64)).cuda()
     else:
-        keys0 = torch.randint(-(1 << 31), (1 << 31) - 1, (n,), dtype=torch.int32,
-                              device="cuda", generator=g)
+        host_keys = pcg32_stream(42, 54, n)
+        keys0 = torch.from_numpy(host_keys.view(np.int32)).cuda()
+    del host_keys
     vals0 = torch.arange(n, dtype=torch.int32, device="cuda") if values else None
     keys = torch.empty_like(keys0)
     vals = torch.empty_like(vals0) if values else None
@@ -247,11 +296,14 @@ LEARNED_DISTS = {
 def learned_leg(torch, clc_hip, ctx, log2n, steps, warmup):
     n = 1 << log2n
     kb, vb = 8, 4
-    # learned: id histogram (k), two id passes and the LDS finish (2(k+v)
-    # each); plain: histogram (k) and 8 passes (2(k+v) each)
-    lbytes = kb + 6 * (kb + vb)
+    # Algorithmic bytes of the learned form: inference + bucket histogram
+    # read (k), one scatter into the buckets (2(k+v)), the LDS finish
+    # (2(k+v)): k + 4(k+v). Executed here: the id histogram (k), two id
+    # passes and the LDS finish (2(k+v) each): k + 6(k+v).
+    lbytes = kb + 4 * (kb + vb)
+    xbytes = kb + 6 * (kb + vb)
     out = {"n": n, "key": "u64", "value": "u32 index",
-           "algorithmic_bytes_per_pair": {"learned": lbytes,
+           "algorithmic_bytes_per_pair": {"learned": lbytes, "learned_executed": xbytes,
                                           "plain": kb + 8 * 2 * (kb + vb)}}
     g = torch.Generator(device="cuda")
     for i, (name, gen) in enumerate(LEARNED_DISTS.items()):
@@ -305,6 +357,8 @@ def learned_leg(torch, clc_hip, ctx, log2n, steps, warmup):
             "speedup": round(res["plain"] / res["learned"], 3),
             "learned_roofline_frac": round(lbytes * n / (res["learned"] / 1e3)
                                            / 1e9 / HBM_PEAK_GBS, 4),
+            "learned_executed_bytes_frac": round(xbytes * n / (res["learned"] / 1e3)
+                                                 / 1e9 / HBM_PEAK_GBS, 4),
             "bit_identical_to_plain": same,
             "used_model": st["used_model"], "num_buckets": st["num_buckets"],
             "sample_size": st["sample_size"],
@@ -435,10 +489,7 @@ def segmented_leg(torch, clc_hip, ctx, steps, warmup):
     np.cumsum(sizes, out=off[1:])
     total = int(off[-1])
     say("segmented leg", nseg, "segments,", total, "pairs")
-    g = torch.Generator(device="cuda")
-    g.manual_seed(77)
-    keys0 = torch.randint(-(1 << 31), (1 << 31) - 1, (total,), dtype=torch.int32,
-                          device="cuda", generator=g)
+    keys0 = torch.from_numpy(pcg32_stream(42, 57, total).view(np.int32)).cuda()
     off_d = torch.from_numpy(off).cuda()
     seg_of = torch.repeat_interleave(torch.arange(nseg, device="cuda"),
                                      torch.from_numpy(sizes).cuda())
@@ -495,6 +546,17 @@ def segmented_leg(torch, clc_hip, ctx, steps, warmup):
     per_seg_wall = time.perf_counter() - t0
     same_per_seg = bool(torch.equal(keys, kg)) and bool(torch.equal(vals, vg))
     graph.close()
+    # Algorithmic bytes by segment (8-byte pairs): an LDS-resident segment
+    # reads and writes its pairs once, 2(k+v); a segment the size-class
+    # table sends to the onesweep path costs the plain sort's k + P*2(k+v)
+    # with P = 4 digit places; one-key segments move nothing.
+    table = ctx.segmented_sort_table(clc_hip.U32, True)
+    large_class = 7
+    buckets = np.minimum(np.ceil(np.log2(np.maximum(sizes, 2))).astype(np.int64), 31)
+    on_large = np.array([table[b] == large_class for b in buckets]) & (sizes >= 2)
+    lds_pairs = int(sizes[(sizes >= 2) & ~on_large].sum())
+    large_pairs = int(sizes[on_large].sum())
+    mixed = lds_pairs * 2 * (4 + 4) + large_pairs * (4 + 4 * 2 * (4 + 4))
     algo = total * 2 * (4 + 4)
     out = {
         "segments": nseg, "pairs": total, "min_len": int(sizes.min()),
@@ -504,13 +566,69 @@ def segmented_leg(torch, clc_hip, ctx, steps, warmup):
         "per_segment_wall_s": round(per_seg_wall, 3),
         "speedup_vs_per_segment": round(per_seg_ms / graph_ms, 1),
         "graph_gkeys": round(total / graph_ms / 1e6, 3),
-        "graph_lds_roofline_frac": round(algo / (graph_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        "graph_roofline_frac": round(mixed / (graph_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        "algorithmic_bytes": mixed,
+        "pairs_in_lds_segments": lds_pairs, "pairs_in_onesweep_segments": large_pairs,
+        "segments_on_onesweep": int(on_large.sum()),
+        "graph_single_pass_frac": round(algo / (graph_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
         "bit_identical": same_direct and same_per_seg,
     }
     say("segmented", out)
     del keys0, vals0, keys, vals, kg, vg, off_d
     torch.cuda.empty_cache()
     return out
+
+
+def c1_leg(torch, clc_hip, ctx, steps, warmup):
+    """The CPU-runnable configuration: 2^20 raw pcg32 outputs (srandom(42,
+    54)) sorted keys-only ascending, and an exclusive plus-scan of 2^20
+    values pcg32 & 0xFF (srandom(42, 55)). Each call timed by hipEvents on
+    resident inputs (restore copies outside the events). 4 MiB in and out
+    stay in the 256 MB Infinity Cache, so the rates are reported but not
+    judged against the HBM roofline."""
+    import numpy as np
+    n = 1 << 20
+    keys0 = torch.from_numpy(pcg32_stream(42, 54, n).view(np.int32)).cuda()
+    x = torch.from_numpy((pcg32_stream(42, 55, n) & np.uint32(0xFF)).view(np.int32)).cuda()
+    keys = torch.empty_like(keys0)
+    out = torch.empty_like(x)
+
+    def timed(before, call):
+        t = []
+        for it in range(warmup + steps):
+            before()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            call()
+            e1.record()
+            torch.cuda.synchronize()
+            if it >= warmup:
+                t.append(e0.elapsed_time(e1))
+        return statistics.fmean(t)
+
+    sort_ms = timed(lambda: keys.copy_(keys0),
+                    lambda: ctx.sort(keys, None, key_type=clc_hip.U32))
+    k = keys.to(torch.int64) & 0xFFFFFFFF
+    sort_ok = bool(torch.equal(torch.sort(keys0.to(torch.int64) & 0xFFFFFFFF).values, k))
+    scan_ms = timed(lambda: None,
+                    lambda: ctx.scan(x, out, "(a + b)", "0", True, type_code=clc_hip.U32))
+    d = out[1:].long() - out[:-1].long()
+    scan_ok = int(out[0].item()) == 0 and bool(torch.equal(d, x[:-1].long()))
+    res = {
+        "n": n, "note": "inputs and outputs (4 MiB each) fit the Infinity Cache: "
+                        "reported, not judged on the HBM roofline",
+        "sort_keys_only": {"ms": round(sort_ms, 4),
+                           "gkeys": round(n / sort_ms / 1e6, 3),
+                           "gbs_at_36B_per_key": round(36 * n / sort_ms / 1e6, 1),
+                           "output_check": sort_ok},
+        "exclusive_scan": {"ms": round(scan_ms, 4),
+                           "gelem_s": round(n / scan_ms / 1e6, 3),
+                           "gbs_at_8B_per_elem": round(8 * n / scan_ms / 1e6, 1),
+                           "output_check": scan_ok},
+    }
+    say("c1", res)
+    return res
 
 
 def primitives_leg(torch, clc_hip, ctx, steps, warmup):
@@ -610,8 +728,8 @@ def main() -> int:
                     help="log2 of the key count (default 28 for u32, 27 for u64)")
     ap.add_argument("--key", choices=["u32", "u64"], default="u32",
                     help="u32: full 32-bit keys; u64: keys uniform over [0, 2^48)")
-    ap.add_argument("--legs", default="u64,learned,segmented,batched,primitives",
-                    help="extra workloads to report: comma list of u64, learned, "
+    ap.add_argument("--legs", default="c1,u64,learned,segmented,batched,primitives",
+                    help="extra workloads to report: comma list of c1, u64, learned, "
                          "segmented, batched, primitives, or 'none'")
     ap.add_argument("--keys-only", action="store_true",
                     help="sort keys without values (main line only)")
@@ -626,7 +744,7 @@ def main() -> int:
         return 2
     legs = [] if args.legs == "none" else [x for x in args.legs.split(",") if x]
     for x in legs:
-        if x not in ("u64", "learned", "segmented", "batched", "primitives"):
+        if x not in ("c1", "u64", "learned", "segmented", "batched", "primitives"):
             ap.error(f"unknown leg {x}")
 
     import torch
@@ -654,7 +772,7 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": args.key,
-        "data": "synthetic (torch.randint seeded; values 0..n-1)",
+        "data": "synthetic (seeded pcg32 stream, srandom(42, 54); values 0..n-1)",
         "config": {"workload": (f"stable {'keys-only' if args.keys_only else 'key-value'} "
                                 f"radix sort, 2^{log2n} "
                                 + ("uint64 keys uniform over [0, 2^48)" if wide else
@@ -678,6 +796,10 @@ def main() -> int:
         "legs": {},
     }
     torch.cuda.empty_cache()
+    if "c1" in legs:
+        c1 = c1_leg(torch, clc_hip, ctx, max(10, args.steps), 3)
+        ok = ok and c1["sort_keys_only"]["output_check"] and c1["exclusive_scan"]["output_check"]
+        line["legs"]["c1_2^20"] = c1
     if "u64" in legs and not wide:
         say("leg u64 2^27")
         u = plain_sort(torch, clc_hip, ctx, True, 27, args.steps, args.warmup,
