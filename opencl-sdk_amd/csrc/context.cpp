@@ -1,13 +1,12 @@
 #include "context.hpp"
 #include "rtc.hpp"
+#include "tuning_file.hpp"
 
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <fstream>
 #include <mutex>
-#include <unistd.h>
 
 namespace clc {
 
@@ -230,8 +229,8 @@ std::string tuning_identity(const context& ctx)
 {
     int rt = 0;
     (void)hipRuntimeGetVersion(&rt);
-    return "clc-tuning-v1 " + ctx.facts().arch + " cu" + std::to_string(ctx.facts().cu_count)
-        + " hip" + std::to_string(rt);
+    return ctx.facts().arch + " cu" + std::to_string(ctx.facts().cu_count) + " hip"
+        + std::to_string(rt);
 }
 
 std::string tuning_path(const context& ctx, const std::string& name)
@@ -248,35 +247,14 @@ std::string tuning_path(const context& ctx, const std::string& name)
 bool load_tuning(const context& ctx, const std::string& name, std::vector<int>& out)
 {
     const std::string path = tuning_path(ctx, name);
-    if (path.empty()) return false;
-    std::ifstream f(path);
-    if (!f) return false;
-    std::string header;
-    std::getline(f, header);
-    if (header != tuning_identity(ctx)) return false;
-    std::vector<int> v;
-    int x;
-    while (f >> x) v.push_back(x);
-    if (v.empty()) return false;
-    out = v;
-    return true;
+    return !path.empty() && read_tuning_file(path, tuning_identity(ctx), out);
 }
 
 void store_tuning(const context& ctx, const std::string& name,
                   const std::vector<int>& values)
 {
     const std::string path = tuning_path(ctx, name);
-    if (path.empty()) return;
-    const std::string tmp = path + ".tmp." + std::to_string(::getpid());
-    {
-        std::ofstream f(tmp, std::ios::trunc);
-        if (!f) return;
-        f << tuning_identity(ctx) << "\n";
-        for (std::size_t i = 0; i < values.size(); ++i)
-            f << values[i] << (i + 1 < values.size() ? " " : "\n");
-        if (!f) return;
-    }
-    if (std::rename(tmp.c_str(), path.c_str()) != 0) std::remove(tmp.c_str());
+    if (!path.empty()) (void)write_tuning_file(path, tuning_identity(ctx), values);
 }
 
 hipStream_t context::aux_stream(int i)

@@ -186,12 +186,9 @@ private:
 };
 
 // ---- on-disk cache of tuned launch tables ----------------------------------
-// One text file per table name in the context's cache directory: a header
-// naming the format, device architecture, CU count and HIP runtime version,
-// then the integers. A file whose header does not match this process is
-// ignored (and re-tuned). Writes go to a temporary file renamed into place,
-// so readers never see a partial file. Mirrors the reference's on-disk
-// program cache (lib/src/Compute/ProgramCache.cpp:163-219).
+// One file per table name in the context's cache directory, in the format of
+// tuning_file.hpp (identity: device architecture, CU count, HIP runtime
+// version); a missing, foreign or damaged file is ignored (and re-tuned).
 
 //! Reads table name; false when there is no cache directory, no file, or a
 //! stale/foreign one.
