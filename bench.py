@@ -402,7 +402,7 @@ def c5_keys_gpu(torch, off, g):
     return keys
 
 
-def learned_batched_leg(torch, clc_hip, ctx, steps, warmup):
+def learned_batched_leg(torch, clc_hip, ctx, steps, warmup, baselines=True):
     """4096 float32 segments, log-uniform sizes 2^10..2^22 (~2^31 pairs in
     total), u32 values: the batched learned sort against the plain
     segmented sort on the same GPU, bit-identical outputs."""
@@ -442,19 +442,21 @@ def learned_batched_leg(torch, clc_hip, ctx, steps, warmup):
         say("learned batched", label, round(res[label], 3), "ms")
     same = bool(torch.equal(kl, kp)) and bool(torch.equal(vl, vp))
     # The reference's way: one plain sort call per segment (timed once).
-    kp.copy_(keys0)
-    vp.copy_(vals0)
-    torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for i in range(nseg):
-        b, e = int(off[i]), int(off[i + 1])
-        ctx.sort(kp[b:e], vp[b:e], key_type=clc_hip.F32)
-    e1.record()
-    torch.cuda.synchronize()
-    res["per_segment"] = e0.elapsed_time(e1)
-    same = same and bool(torch.equal(kl, kp)) and bool(torch.equal(vl, vp))
+    res["per_segment"] = float("nan")
+    if baselines:
+        kp.copy_(keys0)
+        vp.copy_(vals0)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for i in range(nseg):
+            b, e = int(off[i]), int(off[i + 1])
+            ctx.sort(kp[b:e], vp[b:e], key_type=clc_hip.F32)
+        e1.record()
+        torch.cuda.synchronize()
+        res["per_segment"] = e0.elapsed_time(e1)
+        same = same and bool(torch.equal(kl, kp)) and bool(torch.equal(vl, vp))
     kl.copy_(keys0)
     vl.copy_(vals0)
     st = ctx.learned_segmented_sort(kl, vl, off_d, key_type=clc_hip.F32, stats=True)
@@ -479,7 +481,7 @@ def learned_batched_leg(torch, clc_hip, ctx, steps, warmup):
     return out
 
 
-def segmented_leg(torch, clc_hip, ctx, steps, warmup):
+def segmented_leg(torch, clc_hip, ctx, steps, warmup, baselines=True):
     import numpy as np
     nseg = 65536
     rng = np.random.default_rng(2024)
@@ -540,11 +542,14 @@ def segmented_leg(torch, clc_hip, ctx, steps, warmup):
             if st != 0:
                 clc_hip.check(st)
 
-    timed(lambda: per_segment(4096), 1)
-    t0 = time.perf_counter()
-    per_seg_ms = timed(per_segment, 1)[0]
-    per_seg_wall = time.perf_counter() - t0
-    same_per_seg = bool(torch.equal(keys, kg)) and bool(torch.equal(vals, vg))
+    per_seg_ms = per_seg_wall = float("nan")
+    same_per_seg = True
+    if baselines:
+        timed(lambda: per_segment(4096), 1)
+        t0 = time.perf_counter()
+        per_seg_ms = timed(per_segment, 1)[0]
+        per_seg_wall = time.perf_counter() - t0
+        same_per_seg = bool(torch.equal(keys, kg)) and bool(torch.equal(vals, vg))
     graph.close()
     # Algorithmic bytes by segment (8-byte pairs): an LDS-resident segment
     # reads and writes its pairs once, 2(k+v); a segment the size-class
@@ -735,6 +740,10 @@ def main() -> int:
                     help="sort keys without values (main line only)")
     ap.add_argument("--learned-log2n", type=int, default=28)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-baselines", action="store_true",
+                    help="skip the one-sort-call-per-segment baselines of the segmented "
+                         "and batched legs (for profiler runs, which serialise every "
+                         "dispatch)")
     ap.add_argument("--cpu-log2n", type=int, default=28)
     ap.add_argument("--cpu-reps", type=int, default=3)
     args = ap.parse_args()
@@ -818,11 +827,12 @@ def main() -> int:
                         if isinstance(v, dict) and "bit_identical_to_plain" in v)
         line["legs"]["learned"] = lr
     if "segmented" in legs:
-        sg = segmented_leg(torch, clc_hip, ctx, max(3, args.steps // 2), 2)
+        sg = segmented_leg(torch, clc_hip, ctx, max(3, args.steps // 2), 2,
+                           baselines=not args.no_baselines)
         ok = ok and sg["bit_identical"]
         line["legs"]["segmented"] = sg
     if "batched" in legs:
-        lb = learned_batched_leg(torch, clc_hip, ctx, 3, 1)
+        lb = learned_batched_leg(torch, clc_hip, ctx, 3, 1, baselines=not args.no_baselines)
         ok = ok and lb["bit_identical_to_plain"]
         line["legs"]["learned_batched"] = lb
     if "primitives" in legs:

@@ -5,7 +5,7 @@
 Every source under csrc/ is compiled by hipcc for gfx950 into _build/ and
 linked into libclc_hip.so beside this file. hipcc cross-compiles without a
 GPU. --incremental skips sources whose object is newer than the source and
-every header (any header change recompiles everything).
+every file it includes (hipcc writes the list beside each object).
 """
 from __future__ import annotations
 
@@ -74,19 +74,35 @@ def embedded_headers_source() -> str:
 
 def _compile(src: str, extra: list[str]) -> str:
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-    cmd = [HIPCC, *COMMON, *extra, "-c", src, "-o", obj]
+    cmd = [HIPCC, *COMMON, *extra, "-c", src, "-o", obj, "-MMD", "-MF", obj + ".d"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")
     return obj
 
 
-def _fresh(src: str, newest_header: float) -> bool:
+def _deps(obj: str) -> list[str] | None:
+    """Files the object was compiled from (its make-style .d file), or None
+    when the list is missing."""
+    try:
+        with open(obj + ".d") as f:
+            text = f.read().replace("\\\n", " ")
+    except OSError:
+        return None
+    _, _, rest = text.partition(":")
+    return rest.split()
+
+
+def _fresh(src: str) -> bool:
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-    if not os.path.exists(obj):
+    deps = _deps(obj)
+    if not os.path.exists(obj) or deps is None:
         return False
     t = os.path.getmtime(obj)
-    return t > os.path.getmtime(src) and t > newest_header
+    try:
+        return all(os.path.getmtime(d) < t for d in [src, *deps])
+    except OSError:
+        return False
 
 
 def build(jobs: int = 8, verbose: bool = False, incremental: bool = False) -> str:
@@ -95,11 +111,9 @@ def build(jobs: int = 8, verbose: bool = False, incremental: bool = False) -> st
     os.makedirs(BUILD, exist_ok=True)
     extra: list[str] = []
     srcs = sources() + [embedded_headers_source()]
-    headers = glob.glob(os.path.join(CSRC, "*.hpp")) + glob.glob(os.path.join(CSRC, "*.h"))
-    newest = max((os.path.getmtime(h) for h in headers), default=0.0)
 
     def one(s):
-        if incremental and _fresh(s, newest):
+        if incremental and _fresh(s):
             return os.path.join(BUILD, os.path.basename(s) + ".o")
         return _compile(s, extra)
 
@@ -121,7 +135,8 @@ def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=8)
     ap.add_argument("--incremental", action="store_true",
-                    help="recompile only sources older than their object or any header")
+                    help="recompile only sources older than their object or any file "
+                         "they include")
     args = ap.parse_args()
     build(jobs=args.jobs, verbose=True, incremental=args.incremental)
 

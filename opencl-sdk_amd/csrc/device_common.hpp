@@ -53,6 +53,14 @@ __device__ __forceinline__ T fetch_add_agent(T* p, V v)
                                   __HIP_MEMORY_SCOPE_AGENT);
 }
 
+//! Returns *p and stores *p + 1 (wrapping at 2^32), agent scope. Unlike
+//! fetch_add_agent from one lane, the compiler keeps it a single atomic
+//! whose result is waited for only where it is used.
+__device__ __forceinline__ uint32_t inc_agent(uint32_t* p)
+{
+    return __builtin_amdgcn_atomic_inc32(p, 0xFFFFFFFFu, __ATOMIC_RELAXED, "agent");
+}
+
 //! Sum over the wave (every lane gets it).
 __device__ __forceinline__ uint32_t wave_sum(uint32_t x)
 {

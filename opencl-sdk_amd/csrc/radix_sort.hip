@@ -265,7 +265,8 @@ void sort_impl(context& ctx, K* keys, uint32_t* vals, uint32_t n, bool desc, int
         timed_launch timer(ctx, values ? "radix_onesweep_pairs" : "radix_onesweep_keys");
         onesweep_args a{ kb[0], kb[1], vb[0], vb[1], n, uint32_t(p), trivial,
                          base + p * RADIX, st[p % 2],
-                         p + 1 < P ? st[(p + 1) % 2] : nullptr, ctr + p };
+                         p + 1 < P ? st[(p + 1) % 2] : nullptr, ctr + p,
+                         uint32_t(ctx.facts().cu_count) };
         launch_onesweep<K, KIND>(shape, desc, values, wide, a, s);
     }
     const uint32_t copy_grid =

@@ -59,7 +59,8 @@ __global__ __launch_bounds__(256) void classify_kernel(
             if (mask == 0) continue;
             const uint32_t below = count_below(mask);
             uint32_t first = 0;
-            if (c == k && below == 0) first = atomicAdd(&counts[k], uint32_t(__popcll(mask)));
+            if (c == k && below == 0)
+                first = atomicAdd(&counts[k * CTR_STRIDE], uint32_t(__popcll(mask)));
             first = __shfl(first, __ffsll(static_cast<unsigned long long>(mask)) - 1, WAVE);
             if (c != k) continue;
             const uint32_t pos = first + below;
@@ -627,7 +628,7 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
     const std::size_t T_max = L_max ? (num_items + TILE - 1) / TILE + L_max : 0;
 
     layout Lt;
-    const std::size_t o_ctr = Lt.take(256);
+    const std::size_t o_ctr = Lt.take(CTR_SLOTS * CTR_STRIDE * 4);
     const std::size_t o_stA = Lt.take(T_max * RADIX * 4);
     const std::size_t o_hist = Lt.take(L_max * P * RADIX * 4);
     const std::size_t zero_bytes = Lt.bytes;
@@ -668,22 +669,23 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
     temp_buffer tmp(ctx.resource(), Lt.bytes, ctx.stream());
     char* base_ptr = tmp.as<char>();
     uint32_t* ctr = reinterpret_cast<uint32_t*>(base_ptr + o_ctr);
-    // counters: [0, MID] list counts (LARGE's at LARGE, MID's at MID),
-    // 16 histogram queue, 17 tiles in use, 18 large segments in use,
-    // 19 copy-back queue, 20 active large segments, 21 their tiles,
-    // [24, 24 + P) onesweep queues, [40, 40 + P) places that some large
-    // segment runs, [48, 48 + NCFG] LDS-class queues (MID's last)
-    static_assert(MID < 16 && 48 + NCFG + 1 <= 64, "counter layout");
-    uint32_t* counts = ctr;
-    uint32_t* lds_q = ctr + 48;
-    uint32_t* hist_q = ctr + 16;
-    uint32_t* total_tiles = ctr + 17;
-    uint32_t* large_used = ctr + 18;
-    uint32_t* copy_q = ctr + 19;
-    uint32_t* act_used = ctr + 20;
-    uint32_t* act_tiles = ctr + 21;
-    uint32_t* pass_q = ctr + 24;
-    uint32_t* any_exec = ctr + 40;
+    // Counters, one slot of CTR_STRIDE words each (device-scope atomics on
+    // words that share a slot serialise): slots [0, MID] list counts
+    // (LARGE's at LARGE, MID's at MID), 16 histogram queue, 17 tiles in
+    // use, 18 large segments in use, 19 copy-back queue, 20 active large
+    // segments, 21 their tiles, [24, 24 + P) onesweep queues, [48, 48 +
+    // NCFG] LDS-class queues (MID's last); slot 40 holds the P flags of the
+    // places that some large segment runs, side by side (set once each).
+    static_assert(MID < 16 && 48 + NCFG + 1 <= CTR_SLOTS && 24 + 8 <= 40, "counter layout");
+    auto slot = [&](int i) { return ctr + std::size_t(i) * CTR_STRIDE; };
+    uint32_t* counts = slot(0); // class k's count at counts[k * CTR_STRIDE]
+    uint32_t* hist_q = slot(16);
+    uint32_t* total_tiles = slot(17);
+    uint32_t* large_used = slot(18);
+    uint32_t* copy_q = slot(19);
+    uint32_t* act_used = slot(20);
+    uint32_t* act_tiles = slot(21);
+    uint32_t* any_exec = slot(40);
     uint32_t* exec = reinterpret_cast<uint32_t*>(base_ptr + o_exec);
     uint4* entries = reinterpret_cast<uint4*>(base_ptr + o_entries);
     uint32_t* llist = reinterpret_cast<uint32_t*>(base_ptr + o_llist);
@@ -705,7 +707,7 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
     // The large path (segments above every LDS capacity), on stream ls.
     auto large_path = [&](hipStream_t ls) {
         hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, ls, off, llist,
-                           counts + LARGE, static_cast<uint32_t>(L_max),
+                           slot(LARGE), static_cast<uint32_t>(L_max),
                            static_cast<uint32_t>(T_max), TILE, tstart, large_used, total_tiles);
         hipLaunchKernelGGL(hk, dim3(hgrid), dim3(256), 0, ls, keys, off, llist, tstart,
                            large_used, total_tiles, hist, hist_q);
@@ -724,7 +726,7 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
             hipLaunchKernelGGL(ok, dim3(ogrid), dim3(LARGE_BLOCK), olds, ls, kb[0], kb[1],
                                vb[0], vb[1], off, llist, active, astart, act_used,
                                act_tiles, uint32_t(p), dbase, exec, any_exec, st[p % 2],
-                               p + 1 < P ? st[(p + 1) % 2] : nullptr, pass_q + p);
+                               p + 1 < P ? st[(p + 1) % 2] : nullptr, slot(24 + p));
         }
         hipLaunchKernelGGL(ck, dim3(cpgrid), dim3(256), 0, ls, kb[0], kb[1], vb[0], vb[1],
                            off, llist, active, astart, act_used, act_tiles, exec,
@@ -766,14 +768,14 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
         if (used[c])
             run_branch([&](hipStream_t) {
                 launch_lds<K, KIND, DESC, VALUES>(ctx, false, c, keys, vals,
-                                                  entries + std::size_t(c) * nseg, counts + c,
-                                                  lds_q + c);
+                                                  entries + std::size_t(c) * nseg, slot(c),
+                                                  slot(48 + c));
             });
     if (use_mid)
         run_branch([&](hipStream_t) {
             launch_lds_shape<K, KIND, DESC, VALUES, MID_BLOCK, MID_ITEMS>(
-                ctx, false, keys, vals, entries + std::size_t(NCFG) * nseg, counts + MID,
-                lds_q + NCFG);
+                ctx, false, keys, vals, entries + std::size_t(NCFG) * nseg, slot(MID),
+                slot(48 + NCFG));
         });
     if (L_max > 0) large_path(s);
     CLC_HIP(hipGetLastError());

@@ -55,6 +55,11 @@ constexpr uint32_t MID_CAP = MID_BLOCK * MID_ITEMS;
 constexpr int MID_BUCKET = 13;
 //! Segment-size buckets: bucket b holds 2^(b-1) < m <= 2^b (m >= 2).
 constexpr int NBUCKET = 32;
+//! The sort's work counters sit CTR_STRIDE words (256 bytes) apart, one
+//! per slot, CTR_SLOTS slots: device-scope atomics on neighbouring words of
+//! one line are serialised with each other.
+constexpr uint32_t CTR_STRIDE = 64;
+constexpr int CTR_SLOTS = 64;
 //! Onesweep tile of the large path.
 constexpr int LARGE_BLOCK = 1024;
 //! Keys per thread of a large-path onesweep tile: as many as the persistent

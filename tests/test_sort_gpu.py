@@ -211,7 +211,7 @@ def test_unaligned_pointer(ctx):
     assert host(kd, np.uint32)[0] == keys[0]
 
 
-@pytest.mark.parametrize("shape", range(8))
+@pytest.mark.parametrize("shape", range(11))
 @pytest.mark.parametrize("dtype,descending", [(np.uint32, False), (np.float32, True),
                                               (np.uint64, False), (np.int64, True)])
 def test_every_tile_shape(ctx, monkeypatch, shape, dtype, descending):
@@ -230,7 +230,7 @@ def test_every_tile_shape(ctx, monkeypatch, shape, dtype, descending):
 def test_sort_tuner_picks_a_valid_shape(ctx):
     say("sort tuner u32 pairs")
     shape = ctx.sort_tune(clc_hip.U32, True)
-    assert 0 <= shape < 8
+    assert 0 <= shape < 11
     n = 100003
     keys = oracle.pcg32_u32(77, 1, n)
     vals = np.arange(n, dtype=np.uint32)

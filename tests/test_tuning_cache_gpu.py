@@ -125,7 +125,7 @@ def test_sort_shape_stored_and_reused(tmp_path):
     header, line, _ = open(files[0]).read().splitlines()
     assert [int(x) for x in line.split()] == [shape]
     # Every stored shape gives the oracle's output through a new context.
-    for planted in range(8):
+    for planted in range(11):
         write_table(files[0], header, [planted])
         b = clc_hip.Context(cache_dir=cache)
         try:
