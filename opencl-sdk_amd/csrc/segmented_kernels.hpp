@@ -76,45 +76,31 @@ __global__ __launch_bounds__(256) void classify_kernel(
     }
 }
 
-//! Persistent: each workgroup takes segments of its class (entries from
-//! classify_kernel) and sorts each wholly in LDS. Thread 0 claims the next
-//! queue slot while the current segment is sorted, so only the read of
-//! the next entry is exposed between segments.
+//! Persistent: workgroup w sorts the entries w, w + grid, w + 2 grid, ... of
+//! its class (from classify_kernel), each wholly in LDS. The segments of a
+//! class are within a factor of two in size, so the fixed stride balances
+//! the work; a shared queue instead cost one device-scope atomic per
+//! segment on one address, and those serialise (about 20,000 claims for
+//! the smallest class of the 65,536 lognormal segments took longer than
+//! the sorting). The entry is read by every thread from the same address
+//! (a scalar load), so no LDS broadcast and no barrier is needed for it.
 template <typename K, int KIND, bool DESC, bool VALUES, int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void lds_class_kernel(
-    K* keys, uint32_t* vals, const uint4* __restrict__ entries, const uint32_t* count,
-    uint32_t* queue)
+    K* keys, uint32_t* vals, const uint4* __restrict__ entries, const uint32_t* count)
 {
     constexpr int W = BLOCK / WAVE;
     extern __shared__ __align__(16) unsigned char smem[];
     uint32_t* s_hist = reinterpret_cast<uint32_t*>(smem);
     void* s_buf = smem + W * 1024; // TILE * lds_segment_bytes_per_key bytes
     __shared__ uint32_t s_tmp[4];
-    __shared__ uint4 s_entry;
-    __shared__ uint32_t s_ok;
     const uint32_t cnt = *count;
-    uint32_t next = 0; // thread 0: the claimed queue slot
-    if (threadIdx.x == 0) next = atomicAdd(queue, 1u);
-    for (;;)
+    for (uint32_t i = blockIdx.x; i < cnt; i += gridDim.x)
     {
-        if (threadIdx.x == 0)
-        {
-            const bool ok = next < cnt;
-            s_ok = ok;
-            if (ok)
-            {
-                s_entry = entries[next];
-                next = atomicAdd(queue, 1u);
-            }
-        }
-        __syncthreads();
-        const uint4 e = s_entry;
-        const bool ok = s_ok;
-        __syncthreads(); // s_entry / s_ok are rewritten next round
-        if (!ok) break;
+        const uint4 e = entries[i];
         const uint64_t b = e.x | (static_cast<uint64_t>(e.y) << 32);
         lds_sort_segment<K, KIND, DESC, VALUES, BLOCK, ITEMS>(
             keys + b, VALUES ? vals + b : nullptr, e.z, s_hist, s_buf, s_tmp);
+        __syncthreads(); // the next segment reuses the LDS
     }
 }
 
@@ -178,33 +164,27 @@ __device__ __forceinline__ uint32_t find_segment(const uint32_t* tile_start,
     return lo;
 }
 
-//! Persistent: per large segment, counts of every digit place.
+//! Persistent: per large segment, counts of every digit place. Workgroup
+//! w takes tiles w, w + grid, ... (no shared queue: its device-scope
+//! atomics on one address serialise, and the order does not matter here).
 template <typename K, int KIND, bool DESC>
 __global__ __launch_bounds__(256) void large_hist_kernel(
     const K* __restrict__ keys, const uint64_t* __restrict__ off,
     const uint32_t* __restrict__ large_list, const uint32_t* __restrict__ tile_start,
-    const uint32_t* large_used, const uint32_t* total_tiles,
-    uint32_t* __restrict__ hist, uint32_t* queue)
+    const uint32_t* large_used, const uint32_t* total_tiles, uint32_t* __restrict__ hist)
 {
     constexpr int P = sizeof(K);
     constexpr uint32_t TILE = large_tile(sizeof(K));
     constexpr int C = 32 / P; // interleaved copies per bin, as in the plain histogram
     __shared__ uint32_t s_h[P * RADIX * C];
-    __shared__ uint32_t s_item, s_j;
+    __shared__ uint32_t s_j;
     const uint32_t L = *large_used, T = *total_tiles;
     const uint32_t copy = threadIdx.x % C;
-    for (;;)
+    for (uint32_t t = blockIdx.x; t < T; t += gridDim.x)
     {
-        if (threadIdx.x == 0)
-        {
-            const uint32_t t = atomicAdd(queue, 1u);
-            s_item = t;
-            if (t < T) s_j = find_segment(tile_start, L, t);
-        }
+        if (threadIdx.x == 0) s_j = find_segment(tile_start, L, t);
         for (uint32_t i = threadIdx.x; i < P * RADIX * C; i += 256) s_h[i] = 0;
         __syncthreads();
-        const uint32_t t = s_item;
-        if (t >= T) break;
         const uint32_t j = s_j;
         const uint32_t sidx = large_list[j];
         const uint64_t b = off[sidx];
@@ -354,24 +334,17 @@ __global__ __launch_bounds__(256) void large_copy_back_kernel(
     const uint32_t* __restrict__ vals_alt, const uint64_t* __restrict__ off,
     const uint32_t* __restrict__ large_list, const uint32_t* __restrict__ active,
     const uint32_t* __restrict__ tile_start, const uint32_t* large_used,
-    const uint32_t* total_tiles, const uint32_t* __restrict__ exec, uint32_t P,
-    uint32_t* queue)
+    const uint32_t* total_tiles, const uint32_t* __restrict__ exec, uint32_t P)
 {
     constexpr uint32_t TILE = large_tile(sizeof(K));
-    __shared__ uint32_t s_item, s_j;
+    __shared__ uint32_t s_j;
     const uint32_t L = *large_used, T = *total_tiles;
-    for (;;)
+    for (uint32_t t = blockIdx.x; t < T; t += gridDim.x) // no queue, as large_hist_kernel
     {
-        if (threadIdx.x == 0)
-        {
-            const uint32_t t = atomicAdd(queue, 1u);
-            s_item = t;
-            if (t < T) s_j = find_segment(tile_start, L, t);
-        }
+        if (threadIdx.x == 0) s_j = find_segment(tile_start, L, t);
         __syncthreads();
-        const uint32_t t = s_item, j = s_j;
+        const uint32_t j = s_j;
         __syncthreads();
-        if (t >= T) break;
         const uint32_t jf = active[j];
         uint32_t odd = 0;
         for (uint32_t p = 0; p < P; ++p) odd ^= exec[jf * P + p];
@@ -536,7 +509,7 @@ uint32_t lds_shape_grid(const context& ctx)
 
 template <typename K, int KIND, bool DESC, bool VALUES, int BLOCK, int ITEMS>
 void launch_lds_shape(context& ctx, bool prepare, K* keys, uint32_t* vals,
-                      const uint4* entries, const uint32_t* count, uint32_t* queue)
+                      const uint4* entries, const uint32_t* count)
 {
     const uint32_t grid = lds_shape_grid<K, KIND, DESC, VALUES, BLOCK, ITEMS>(ctx);
     if (prepare) return;
@@ -545,25 +518,25 @@ void launch_lds_shape(context& ctx, bool prepare, K* keys, uint32_t* vals,
     timed_launch timer(ctx, name.c_str());
     hipLaunchKernelGGL((lds_class_kernel<K, KIND, DESC, VALUES, BLOCK, ITEMS>), dim3(grid),
                        dim3(BLOCK), shape_lds(BLOCK, std::size_t(BLOCK) * ITEMS, sizeof(K), VALUES),
-                       ctx.stream(), keys, vals, entries, count, queue);
+                       ctx.stream(), keys, vals, entries, count);
 }
 
 template <typename K, int KIND, bool DESC, bool VALUES, int C>
 void launch_lds_class(context& ctx, bool prepare, K* keys, uint32_t* vals,
-                      const uint4* entries, const uint32_t* count, uint32_t* queue)
+                      const uint4* entries, const uint32_t* count)
 {
     launch_lds_shape<K, KIND, DESC, VALUES, CFG_BLOCK[C], CFG_ITEMS[C]>(
-        ctx, prepare, keys, vals, entries, count, queue);
+        ctx, prepare, keys, vals, entries, count);
 }
 
 template <typename K, int KIND, bool DESC, bool VALUES>
 void launch_lds(context& ctx, bool prepare, int c, K* keys, uint32_t* vals,
-                const uint4* entries, const uint32_t* count, uint32_t* queue)
+                const uint4* entries, const uint32_t* count)
 {
 #define CLC_LDS_CASE(C)                                                        \
     case C:                                                                    \
         launch_lds_class<K, KIND, DESC, VALUES, C>(ctx, prepare, keys, vals,   \
-                                                   entries, count, queue);     \
+                                                   entries, count);            \
         return;
     switch (c)
     {
@@ -656,10 +629,10 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
         for (int c = 0; c < NCFG; ++c)
             if (used[c])
                 launch_lds<K, KIND, DESC, VALUES>(ctx, true, c, nullptr, nullptr, nullptr,
-                                                  nullptr, nullptr);
+                                                  nullptr);
         if (use_mid)
             launch_lds_shape<K, KIND, DESC, VALUES, MID_BLOCK, MID_ITEMS>(
-                ctx, true, nullptr, nullptr, nullptr, nullptr, nullptr);
+                ctx, true, nullptr, nullptr, nullptr, nullptr);
         // The branch streams and events exist before any capture.
         for (int i = 0; i < context::AUX_STREAMS; ++i) ctx.aux_stream(i);
         for (int i = 0; i < context::AUX_EVENTS; ++i) ctx.aux_event(i);
@@ -671,18 +644,16 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
     uint32_t* ctr = reinterpret_cast<uint32_t*>(base_ptr + o_ctr);
     // Counters, one slot of CTR_STRIDE words each (device-scope atomics on
     // words that share a slot serialise): slots [0, MID] list counts
-    // (LARGE's at LARGE, MID's at MID), 16 histogram queue, 17 tiles in
-    // use, 18 large segments in use, 19 copy-back queue, 20 active large
-    // segments, 21 their tiles, [24, 24 + P) onesweep queues, [48, 48 +
-    // NCFG] LDS-class queues (MID's last); slot 40 holds the P flags of the
-    // places that some large segment runs, side by side (set once each).
-    static_assert(MID < 16 && 48 + NCFG + 1 <= CTR_SLOTS && 24 + 8 <= 40, "counter layout");
+    // (LARGE's at LARGE, MID's at MID), 17 tiles in use, 18 large segments
+    // in use, 20 active large segments, 21 their tiles, [24, 24 + P)
+    // onesweep queues (tiles must start in order there); slot 40 holds
+    // the P flags of the places that some large segment runs, side by side
+    // (set once each).
+    static_assert(MID < 16 && 24 + 8 <= 40 && 40 < CTR_SLOTS, "counter layout");
     auto slot = [&](int i) { return ctr + std::size_t(i) * CTR_STRIDE; };
     uint32_t* counts = slot(0); // class k's count at counts[k * CTR_STRIDE]
-    uint32_t* hist_q = slot(16);
     uint32_t* total_tiles = slot(17);
     uint32_t* large_used = slot(18);
-    uint32_t* copy_q = slot(19);
     uint32_t* act_used = slot(20);
     uint32_t* act_tiles = slot(21);
     uint32_t* any_exec = slot(40);
@@ -710,7 +681,7 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
                            slot(LARGE), static_cast<uint32_t>(L_max),
                            static_cast<uint32_t>(T_max), TILE, tstart, large_used, total_tiles);
         hipLaunchKernelGGL(hk, dim3(hgrid), dim3(256), 0, ls, keys, off, llist, tstart,
-                           large_used, total_tiles, hist, hist_q);
+                           large_used, total_tiles, hist);
         const uint32_t sgrid = static_cast<uint32_t>(std::min<std::size_t>(L_max * P, 4096));
         hipLaunchKernelGGL(large_digit_scan_kernel, dim3(sgrid), dim3(256), 0, ls, hist,
                            dbase, large_used, uint32_t(P), off, llist, exec, any_exec);
@@ -730,7 +701,7 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
         }
         hipLaunchKernelGGL(ck, dim3(cpgrid), dim3(256), 0, ls, kb[0], kb[1], vb[0], vb[1],
                            off, llist, active, astart, act_used, act_tiles, exec,
-                           uint32_t(P), copy_q);
+                           uint32_t(P));
     };
 
     // The LDS classes sort disjoint segments: each is a branch forked from
@@ -768,14 +739,12 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
         if (used[c])
             run_branch([&](hipStream_t) {
                 launch_lds<K, KIND, DESC, VALUES>(ctx, false, c, keys, vals,
-                                                  entries + std::size_t(c) * nseg, slot(c),
-                                                  slot(48 + c));
+                                                  entries + std::size_t(c) * nseg, slot(c));
             });
     if (use_mid)
         run_branch([&](hipStream_t) {
             launch_lds_shape<K, KIND, DESC, VALUES, MID_BLOCK, MID_ITEMS>(
-                ctx, false, keys, vals, entries + std::size_t(NCFG) * nseg, slot(MID),
-                slot(48 + NCFG));
+                ctx, false, keys, vals, entries + std::size_t(NCFG) * nseg, slot(MID));
         });
     if (L_max > 0) large_path(s);
     CLC_HIP(hipGetLastError());
