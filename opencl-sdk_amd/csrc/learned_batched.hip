@@ -187,8 +187,8 @@ __device__ void block_moments(double (&mo)[5], double (*red)[5])
 
 //! Block per learned segment: fits its RMI on the sorted sample and checks
 //! the fit. The root is linear in the 32-bit order image (for floats that
-//! image is already piecewise linear in log2 |x|). Leaf l is fitted by
-//! thread l over its samples (least squares relative to the leaf's first
+//! image is already piecewise linear in log2 |x|). Each leaf is fitted by
+//! one wave over its samples (least squares relative to the leaf's first
 //! sample). The check counts samples i whose bucket also holds sample i + K,
 //! K samples standing for cap keys, with a different key: those keys sit in
 //! buckets too large for LDS with more than one distinct key. Above
@@ -249,14 +249,19 @@ __global__ __launch_bounds__(256) void fit_kernel(const uint32_t* __restrict__ s
     }
     __syncthreads();
 
-    // Leaf tid.
+    // Leaves, one wave each at a time (leaf wave, wave + 4, ...): the lanes
+    // stride over the leaf's samples, the moments are summed over the wave
+    // in a fixed order, lane 0 stores the leaf. (One thread per leaf walked
+    // a large leaf's samples one dependent load after another.)
+    const uint32_t lane = tid % WAVE;
+    for (uint32_t l = tid / WAVE; l < LEAVES; l += 256 / WAVE)
     {
-        const uint32_t l = tid, lo = lstart[l], hi = lstart[l + 1];
+        const uint32_t lo = lstart[l], hi = lstart[l + 1];
         double a = 0, b = (double(lo) + double(hi)) / (2.0 * ns);
         if (hi - lo >= 2)
         {
             double lm[5] = { 0, 0, 0, 0, 0 };
-            for (uint32_t i = lo; i < hi; ++i)
+            for (uint32_t i = lo + lane; i < hi; i += WAVE)
             {
                 const double x = static_cast<double>(s[i] - s[lo]);
                 const double y = static_cast<double>(i - lo);
@@ -266,18 +271,26 @@ __global__ __launch_bounds__(256) void fit_kernel(const uint32_t* __restrict__ s
                 lm[3] += x * x;
                 lm[4] += x * y;
             }
+#pragma unroll
+            for (int k = 0; k < 5; ++k)
+#pragma unroll
+                for (int o = WAVE / 2; o > 0; o >>= 1) lm[k] += __shfl_xor(lm[k], o, WAVE);
             double ai, ci;
             line_from_moments(lm, ai, ci); // index - lo ~ ai * (x - s[lo]) + ci
             a = ai / double(ns);
             b = (double(lo) + ci - ai * static_cast<double>(s[lo])) / double(ns);
         }
-        const uint32_t blo = static_cast<uint32_t>(uint64_t(nb) * lo / ns);
-        const uint32_t next = l + 1 < LEAVES ? static_cast<uint32_t>(uint64_t(nb) * hi / ns) : nb - 1;
-        rmi_leaf& L = sm.leaf[l];
-        L.a = a * double(nb); // positions in buckets, not CDF fractions
-        L.b = b * double(nb);
-        L.lo = min(blo, nb - 1);
-        L.hi = max(L.lo, min(next, nb - 1));
+        if (lane == 0)
+        {
+            const uint32_t blo = static_cast<uint32_t>(uint64_t(nb) * lo / ns);
+            const uint32_t next =
+                l + 1 < LEAVES ? static_cast<uint32_t>(uint64_t(nb) * hi / ns) : nb - 1;
+            rmi_leaf& L = sm.leaf[l];
+            L.a = a * double(nb); // positions in buckets, not CDF fractions
+            L.b = b * double(nb);
+            L.lo = min(blo, nb - 1);
+            L.hi = max(L.lo, min(next, nb - 1));
+        }
     }
     __syncthreads();
 
@@ -303,79 +316,109 @@ __global__ __launch_bounds__(256) void fit_kernel(const uint32_t* __restrict__ s
     }
 }
 
-//! Learned segment of tile t: the last segment j with info[j].tile <= t
-//! among learned-capable ones (binary search over segments; segments that
-//! are not learned own no tiles, so equal tile starts resolve to the last).
-__device__ __forceinline__ uint32_t segment_of_tile(const seg_info* info, uint32_t nseg,
-                                                    uint32_t t)
+//! Tile -> segment map of the id passes' tile layout: tile_seg[t] = j for
+//! the tiles of each segment that prepare_kernel made learned (run right
+//! after it, before a fit can clear the flag: the layout stays). One load
+//! per tile instead of a binary search over the segments, about log2(nseg)
+//! dependent loads spent idle by every tile.
+__global__ __launch_bounds__(256) void tile_map_kernel(const seg_info* __restrict__ info,
+                                                       uint32_t nseg,
+                                                       uint32_t* __restrict__ tile_seg)
 {
-    uint32_t lo = 0, hi = nseg; // answer in [lo, hi)
-    while (hi - lo > 1)
+    for (uint32_t j = blockIdx.x; j < nseg; j += gridDim.x)
     {
-        const uint32_t mid = (lo + hi) / 2;
-        if (info[mid].tile <= t) lo = mid;
-        else hi = mid;
+        const seg_info si = info[j];
+        if (!si.learned) continue;
+        const uint32_t nt = (si.m + BTILE - 1) / BTILE;
+        for (uint32_t k = threadIdx.x; k < nt; k += 256) tile_seg[si.tile + k] = j;
     }
-    return lo;
 }
 
 //! Per learned segment (j), counts of the bucket id's low and high byte:
 //! hist[j][0][d], hist[j][1][d]. Block b walks tiles [b * HU, b * HU + HU)
 //! in order, counting in LDS and adding to global memory when the segment
 //! changes or at the end (one flush per segment piece, not per tile), with
-//! HL independent loads per thread in flight.
-constexpr uint32_t HU = 8, HL = 4;
+//! HL independent loads per thread in flight. Low bytes count into HC
+//! interleaved copies per bin (copy = lane mod HC), so lanes that share a
+//! bucket do not serialise on one LDS word. The high byte takes few values
+//! (it is 0 for segments of at most 256 buckets, whose second pass does not
+//! run, and is not counted there): up to HB_BALLOT values it is counted by
+//! one wave ballot per value and one add per wave, above that like the low
+//! byte.
+constexpr uint32_t HU = 8, HL = 4, HC = 8, HB_BALLOT = 8;
 template <int KIND, bool DESC>
 __global__ __launch_bounds__(1024) void id_hist_kernel(const uint32_t* __restrict__ keys,
                                                        const seg_info* __restrict__ info,
-                                                       uint32_t nseg,
+                                                       const uint32_t* __restrict__ tile_seg,
                                                        const rmi_model* __restrict__ models,
                                                        const uint32_t* totals,
                                                        uint32_t* __restrict__ hist)
 {
     __shared__ rmi_model sm;
-    __shared__ uint32_t s_h[2 * RADIX];
+    __shared__ uint32_t s_h[2 * RADIX * HC];
     const uint32_t T = totals[2];
     const uint32_t t0 = blockIdx.x * HU, t1 = min(T, t0 + HU);
+    const uint32_t lane = threadIdx.x % WAVE, copy = threadIdx.x % HC;
     uint32_t cur = 0xFFFFFFFFu; // segment in LDS
+    auto flush = [&](uint32_t j) {
+        for (uint32_t i = threadIdx.x; i < 2 * RADIX; i += 1024)
+        {
+            uint32_t c = 0;
+#pragma unroll
+            for (uint32_t q = 0; q < HC; ++q) c += s_h[i * HC + q];
+            if (c != 0) atomicAdd(&hist[size_t(j) * 2 * RADIX + i], c);
+        }
+    };
     for (uint32_t t = t0; t < t1; ++t)
     {
-        const uint32_t j = segment_of_tile(info, nseg, t);
+        const uint32_t j = tile_seg[t];
         const seg_info si = info[j];
         if (!si.learned) continue; // uniform: every thread reads the same info
         if (j != cur)
         {
             __syncthreads(); // previous counts complete
-            if (cur != 0xFFFFFFFFu)
-                for (uint32_t i = threadIdx.x; i < 2 * RADIX; i += 1024)
-                    if (s_h[i] != 0) atomicAdd(&hist[size_t(cur) * 2 * RADIX + i], s_h[i]);
+            if (cur != 0xFFFFFFFFu) flush(cur);
             __syncthreads();
-            for (uint32_t i = threadIdx.x; i < 2 * RADIX; i += 1024) s_h[i] = 0;
+            for (uint32_t i = threadIdx.x; i < 2 * RADIX * HC; i += 1024) s_h[i] = 0;
             load_model(&sm, models + j);
             cur = j;
             __syncthreads();
         }
+        const uint32_t hmax = (si.nb - 1) >> 8; // high bytes lie in [0, hmax]
         const uint32_t lo = (t - si.tile) * BTILE, hi = min(si.m, lo + BTILE);
         const uint32_t* in = keys + si.begin;
-        for (uint32_t i = lo + threadIdx.x; i < hi; i += HL * 1024)
+        // Whole waves run every iteration (the ballots need all lanes).
+        for (uint32_t i0 = lo; i0 < hi; i0 += HL * 1024)
         {
+            const uint32_t i = i0 + threadIdx.x;
             uint32_t k[HL];
 #pragma unroll
             for (uint32_t u = 0; u < HL; ++u) k[u] = in[min(i + u * 1024, hi - 1)];
 #pragma unroll
             for (uint32_t u = 0; u < HL; ++u)
-                if (i + u * 1024 < hi)
+            {
+                const bool valid = i + u * 1024 < hi;
+                const uint32_t b = bucket_of(sm, order_bits<uint32_t, KIND, DESC>(k[u]));
+                if (valid) atomicAdd(&s_h[(b & 0xFFu) * HC + copy], 1u);
+                if (hmax >= HB_BALLOT)
                 {
-                    const uint32_t b = bucket_of(sm, order_bits<uint32_t, KIND, DESC>(k[u]));
-                    atomicAdd(&s_h[b & 0xFFu], 1u);
-                    atomicAdd(&s_h[RADIX + (b >> 8)], 1u);
+                    if (valid) atomicAdd(&s_h[(RADIX + (b >> 8)) * HC + copy], 1u);
                 }
+                else if (hmax > 0)
+                {
+                    const uint32_t hb = valid ? b >> 8 : 0xFFFFFFFFu;
+                    for (uint32_t h = 0; h <= hmax; ++h)
+                    {
+                        const uint64_t mask = __ballot(hb == h);
+                        if (lane == 0 && mask != 0)
+                            atomicAdd(&s_h[(RADIX + h) * HC], static_cast<uint32_t>(__popcll(mask)));
+                    }
+                }
+            }
         }
     }
     __syncthreads();
-    if (cur != 0xFFFFFFFFu)
-        for (uint32_t i = threadIdx.x; i < 2 * RADIX; i += 1024)
-            if (s_h[i] != 0) atomicAdd(&hist[size_t(cur) * 2 * RADIX + i], s_h[i]);
+    if (cur != 0xFFFFFFFFu) flush(cur);
 }
 
 //! Block per (segment, place): exclusive scan of the 256 counts.
@@ -396,8 +439,8 @@ template <int KIND, bool DESC, bool VALUES>
 __global__ __launch_bounds__(BBLOCK) void id_onesweep_kernel(
     const uint32_t* __restrict__ keys_in, uint32_t* __restrict__ keys_out,
     const uint32_t* __restrict__ vals_in, uint32_t* __restrict__ vals_out,
-    const seg_info* __restrict__ info, uint32_t nseg, const rmi_model* __restrict__ models,
-    const uint32_t* totals, uint32_t pass, const uint32_t* __restrict__ base,
+    const seg_info* __restrict__ info, const uint32_t* __restrict__ tile_seg,
+    const rmi_model* __restrict__ models, const uint32_t* totals, uint32_t pass, const uint32_t* __restrict__ base,
     uint32_t* status, uint32_t* status_next, uint32_t* tile_counter)
 {
     constexpr int W = BBLOCK / WAVE;
@@ -416,7 +459,7 @@ __global__ __launch_bounds__(BBLOCK) void id_onesweep_kernel(
     const uint32_t t = s_tile;
     if (t >= totals[2]) return; // uniform
     if (status_next != nullptr && tid < RADIX) status_next[size_t(t) * RADIX + tid] = 0;
-    const uint32_t j = segment_of_tile(info, nseg, t);
+    const uint32_t j = tile_seg[t];
     const seg_info si = info[j];
     if (!si.learned || (pass == 1 && si.nb <= RADIX)) return; // this pass leaves it as is
     for (uint32_t i = tid; i < W * RADIX; i += BBLOCK) s_hist[i] = 0;
@@ -445,11 +488,12 @@ template <bool VALUES>
 __global__ __launch_bounds__(256) void copy_back_kernel(
     uint32_t* __restrict__ keys, const uint32_t* __restrict__ keys_alt,
     uint32_t* __restrict__ vals, const uint32_t* __restrict__ vals_alt,
-    const seg_info* __restrict__ info, uint32_t nseg, const uint32_t* totals)
+    const seg_info* __restrict__ info, const uint32_t* __restrict__ tile_seg,
+    const uint32_t* totals)
 {
     const uint32_t t = blockIdx.x;
     if (t >= totals[2]) return;
-    const uint32_t j = segment_of_tile(info, nseg, t);
+    const uint32_t j = tile_seg[t];
     const seg_info si = info[j];
     if (!si.learned || si.nb > RADIX) return;
     const uint32_t lo = (t - si.tile) * BTILE, hi = min(si.m, lo + BTILE);
@@ -581,6 +625,7 @@ void run(context& ctx, uint32_t* keys, uint32_t* vals, std::size_t n, const uint
     const std::size_t o_samples = L.take(samples_max * 4);
     const std::size_t o_models = L.take(std::size_t(nseg) * sizeof(rmi_model));
     const std::size_t o_out = L.take((slots_max + 1) * 8);
+    const std::size_t o_tseg = L.take(tiles_max * 4);
     const std::size_t o_kalt = L.take(n * 4);
     const std::size_t o_valt = VALUES ? L.take(n * 4) : 0;
     temp_buffer tmp(ctx.resource(), L.bytes, ctx.stream());
@@ -597,6 +642,7 @@ void run(context& ctx, uint32_t* keys, uint32_t* vals, std::size_t n, const uint
     uint32_t* samples = reinterpret_cast<uint32_t*>(t + o_samples);
     rmi_model* models = reinterpret_cast<rmi_model*>(t + o_models);
     uint64_t* bout = reinterpret_cast<uint64_t*>(t + o_out);
+    uint32_t* tseg = reinterpret_cast<uint32_t*>(t + o_tseg);
     uint32_t* kalt = reinterpret_cast<uint32_t*>(t + o_kalt);
     uint32_t* valt = VALUES ? reinterpret_cast<uint32_t*>(t + o_valt) : nullptr;
 
@@ -605,6 +651,8 @@ void run(context& ctx, uint32_t* keys, uint32_t* vals, std::size_t n, const uint
         timed_launch timer(ctx, "learned_batched_fit");
         hipLaunchKernelGGL(prepare_kernel, dim3(1), dim3(1024), 0, s, off, nseg,
                            static_cast<uint64_t>(n), cap, info, soff, totals);
+        hipLaunchKernelGGL(tile_map_kernel, dim3(std::min<uint32_t>(nseg, 4096)), dim3(256), 0,
+                           s, info, nseg, tseg);
         hipLaunchKernelGGL((sample_kernel<KIND, DESC>), dim3(nseg), dim3(256), 0, s, keys, info,
                            soff, samples);
         detail::segmented_sort(ctx, samples, TY_U32, nullptr, samples_max, soff, nseg, false);
@@ -615,7 +663,7 @@ void run(context& ctx, uint32_t* keys, uint32_t* vals, std::size_t n, const uint
     {
         timed_launch timer(ctx, "learned_batched_id_histogram");
         hipLaunchKernelGGL((id_hist_kernel<KIND, DESC>), dim3(tgrid / HU + 1), dim3(1024), 0, s,
-                           keys, info, nseg, models, totals, hist);
+                           keys, info, tseg, models, totals, hist);
     }
     hipLaunchKernelGGL(id_scan_kernel, dim3(nseg * 2), dim3(256), 0, s, hist, base);
     const std::size_t lds = std::size_t(BBLOCK / WAVE) * 1024 + RADIX * 4
@@ -633,15 +681,15 @@ void run(context& ctx, uint32_t* keys, uint32_t* vals, std::size_t n, const uint
     {
         timed_launch timer(ctx, "learned_batched_id_onesweep");
         hipLaunchKernelGGL(kernel, dim3(tgrid), dim3(BBLOCK), lds, s, keys, kalt, vals, valt,
-                           info, nseg, models, totals, 0u, base, st[0], st[1], ctr);
+                           info, tseg, models, totals, 0u, base, st[0], st[1], ctr);
     }
     {
         timed_launch timer(ctx, "learned_batched_id_onesweep");
         hipLaunchKernelGGL(kernel, dim3(tgrid), dim3(BBLOCK), lds, s, kalt, keys, valt, vals,
-                           info, nseg, models, totals, 1u, base, st[1], nullptr, ctr + 1);
+                           info, tseg, models, totals, 1u, base, st[1], nullptr, ctr + 1);
     }
     hipLaunchKernelGGL((copy_back_kernel<VALUES>), dim3(tgrid), dim3(256), 0, s, keys, kalt, vals,
-                       valt, info, nseg, totals);
+                       valt, info, tseg, totals);
     {
         timed_launch timer(ctx, "learned_batched_bounds");
         hipLaunchKernelGGL((bounds_kernel<KIND, DESC>), dim3(slots_max / 256 + 1), dim3(256), 0,

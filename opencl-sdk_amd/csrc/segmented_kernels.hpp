@@ -150,46 +150,73 @@ __global__ __launch_bounds__(1024) void tile_scan_kernel(
     }
 }
 
-//! Large segment j holding global tile t: tile_start[j] <= t < tile_start[j+1].
-__device__ __forceinline__ uint32_t find_segment(const uint32_t* tile_start,
-                                                 uint32_t L, uint32_t t)
+//! Tile layout -> segment map: map[t] = j for the tiles t of segment j
+//! (start[j] <= t < start[j + 1], t < *total), so a tile finds its segment
+//! with one load. (A binary search is about log2(count) dependent loads,
+//! a few microseconds that a workgroup holding one tile per CU spends
+//! idle on every tile.)
+__global__ __launch_bounds__(256) void tile_map_kernel(const uint32_t* __restrict__ start,
+                                                       const uint32_t* count,
+                                                       const uint32_t* total,
+                                                       uint32_t* __restrict__ map)
 {
-    uint32_t lo = 0, hi = L; // answer in [lo, hi)
-    while (hi - lo > 1)
+    const uint32_t L = *count, T = *total;
+    for (uint32_t j = blockIdx.x; j < L; j += gridDim.x)
     {
-        const uint32_t mid = (lo + hi) / 2;
-        if (tile_start[mid] <= t) lo = mid;
-        else hi = mid;
+        const uint32_t b = start[j], e = min(start[j + 1], T);
+        for (uint32_t t = b + threadIdx.x; t < e; t += 256) map[t] = j;
     }
-    return lo;
 }
 
 //! Persistent: per large segment, counts of every digit place. Workgroup
 //! w takes tiles w, w + grid, ... (no shared queue: its device-scope
-//! atomics on one address serialise, and the order does not matter here).
+//! atomics on one address serialise, and the order does not matter here),
+//! keeps counting in LDS while its tiles stay in one segment and adds the
+//! counts to global memory when the segment changes and at the end (not
+//! once per tile: every tile of a big segment would add to the same
+//! P * 256 words). Each thread has U key loads in flight.
 template <typename K, int KIND, bool DESC>
 __global__ __launch_bounds__(256) void large_hist_kernel(
     const K* __restrict__ keys, const uint64_t* __restrict__ off,
     const uint32_t* __restrict__ large_list, const uint32_t* __restrict__ tile_start,
-    const uint32_t* large_used, const uint32_t* total_tiles, uint32_t* __restrict__ hist)
+    const uint32_t* __restrict__ tile_seg, const uint32_t* total_tiles,
+    uint32_t* __restrict__ hist)
 {
     constexpr int P = sizeof(K);
     constexpr uint32_t TILE = large_tile(sizeof(K));
     constexpr int C = 32 / P; // interleaved copies per bin, as in the plain histogram
+    constexpr uint32_t U = 8;
     __shared__ uint32_t s_h[P * RADIX * C];
-    __shared__ uint32_t s_j;
-    const uint32_t L = *large_used, T = *total_tiles;
-    const uint32_t copy = threadIdx.x % C;
+    const uint32_t T = *total_tiles;
+    const uint32_t tid = threadIdx.x, copy = tid % C;
+    uint32_t cur = 0xFFFFFFFFu, first = 0;
+    uint64_t b = 0, m = 0;
+    auto flush = [&]() {
+        for (uint32_t i = tid; i < P * RADIX; i += 256)
+        {
+            uint32_t c = 0;
+#pragma unroll
+            for (int q = 0; q < C; ++q) c += s_h[i * C + q];
+            if (c != 0) atomicAdd(&hist[size_t(cur) * P * RADIX + i], c);
+        }
+    };
     for (uint32_t t = blockIdx.x; t < T; t += gridDim.x)
     {
-        if (threadIdx.x == 0) s_j = find_segment(tile_start, L, t);
-        for (uint32_t i = threadIdx.x; i < P * RADIX * C; i += 256) s_h[i] = 0;
-        __syncthreads();
-        const uint32_t j = s_j;
-        const uint32_t sidx = large_list[j];
-        const uint64_t b = off[sidx];
-        const uint64_t m = off[sidx + 1] - b;
-        const uint64_t lo = uint64_t(t - tile_start[j]) * TILE;
+        const uint32_t j = tile_seg[t]; // uniform
+        if (j != cur)
+        {
+            __syncthreads(); // the previous segment's counts are complete
+            if (cur != 0xFFFFFFFFu) flush();
+            __syncthreads();
+            for (uint32_t i = tid; i < P * RADIX * C; i += 256) s_h[i] = 0;
+            cur = j;
+            const uint32_t sidx = large_list[j];
+            b = off[sidx];
+            m = off[sidx + 1] - b;
+            first = tile_start[j];
+            __syncthreads();
+        }
+        const uint64_t lo = uint64_t(t - first) * TILE;
         const uint64_t hi = std::min<uint64_t>(lo + TILE, m);
         // Digits equal to the tile's first key's (a place constant across
         // the segment, common for large segments) are counted in registers
@@ -202,33 +229,34 @@ __global__ __launch_bounds__(256) void large_hist_kernel(
             dom[p] = digit_of<K, KIND, DESC>(k0, 8u * p);
             hits[p] = 0;
         }
-        for (uint64_t i = lo + threadIdx.x; i < hi; i += 256)
+        for (uint64_t i0 = lo; i0 < hi; i0 += 256 * U)
         {
-            const K k = keys[b + i];
+            K k[U];
 #pragma unroll
-            for (int p = 0; p < P; ++p)
+            for (uint32_t u = 0; u < U; ++u)
+                k[u] = keys[b + std::min<uint64_t>(i0 + u * 256 + tid, hi - 1)]; // unconditional
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u)
             {
-                const uint32_t d = digit_of<K, KIND, DESC>(k, 8u * p);
-                if (d == dom[p]) ++hits[p];
-                else atomicAdd(&s_h[(p * RADIX + d) * C + copy], 1u);
+                if (i0 + u * 256 + tid >= hi) break;
+#pragma unroll
+                for (int p = 0; p < P; ++p)
+                {
+                    const uint32_t d = digit_of<K, KIND, DESC>(k[u], 8u * p);
+                    if (d == dom[p]) ++hits[p];
+                    else atomicAdd(&s_h[(p * RADIX + d) * C + copy], 1u);
+                }
             }
         }
 #pragma unroll
         for (int p = 0; p < P; ++p)
         {
             const uint32_t h = wave_sum(hits[p]);
-            if (threadIdx.x % WAVE == 0 && h != 0) atomicAdd(&s_h[(p * RADIX + dom[p]) * C], h);
+            if (tid % WAVE == 0 && h != 0) atomicAdd(&s_h[(p * RADIX + dom[p]) * C], h);
         }
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < P * RADIX; i += 256)
-        {
-            uint32_t c = 0;
-#pragma unroll
-            for (int q = 0; q < C; ++q) c += s_h[i * C + q];
-            if (c != 0) atomicAdd(&hist[size_t(j) * P * RADIX + i], c);
-        }
-        __syncthreads();
     }
+    __syncthreads();
+    if (cur != 0xFFFFFFFFu) flush();
 }
 
 //! Persistent: exclusive scan of each (large segment, digit place) count
@@ -333,18 +361,14 @@ __global__ __launch_bounds__(256) void large_copy_back_kernel(
     K* __restrict__ keys, const K* __restrict__ keys_alt, uint32_t* __restrict__ vals,
     const uint32_t* __restrict__ vals_alt, const uint64_t* __restrict__ off,
     const uint32_t* __restrict__ large_list, const uint32_t* __restrict__ active,
-    const uint32_t* __restrict__ tile_start, const uint32_t* large_used,
+    const uint32_t* __restrict__ tile_start, const uint32_t* __restrict__ tile_seg,
     const uint32_t* total_tiles, const uint32_t* __restrict__ exec, uint32_t P)
 {
     constexpr uint32_t TILE = large_tile(sizeof(K));
-    __shared__ uint32_t s_j;
-    const uint32_t L = *large_used, T = *total_tiles;
+    const uint32_t T = *total_tiles;
     for (uint32_t t = blockIdx.x; t < T; t += gridDim.x) // no queue, as large_hist_kernel
     {
-        if (threadIdx.x == 0) s_j = find_segment(tile_start, L, t);
-        __syncthreads();
-        const uint32_t j = s_j;
-        __syncthreads();
+        const uint32_t j = tile_seg[t];
         const uint32_t jf = active[j];
         uint32_t odd = 0;
         for (uint32_t p = 0; p < P; ++p) odd ^= exec[jf * P + p];
@@ -376,7 +400,7 @@ __global__ __launch_bounds__(LARGE_BLOCK) void large_onesweep_kernel(
     K* keys_a, K* keys_b, uint32_t* vals_a, uint32_t* vals_b,
     const uint64_t* __restrict__ off, const uint32_t* __restrict__ large_list,
     const uint32_t* __restrict__ active, const uint32_t* __restrict__ tile_start,
-    const uint32_t* large_used, const uint32_t* total_tiles, uint32_t pass,
+    const uint32_t* __restrict__ tile_seg, const uint32_t* total_tiles, uint32_t pass,
     const uint32_t* __restrict__ base, const uint32_t* __restrict__ exec,
     const uint32_t* __restrict__ any_exec,
     uint32_t* status, uint32_t* status_next, uint32_t* queue)
@@ -393,7 +417,7 @@ __global__ __launch_bounds__(LARGE_BLOCK) void large_onesweep_kernel(
     __shared__ uint32_t s_tmp[4];
     __shared__ uint32_t s_item, s_j;
     const uint32_t tid = threadIdx.x;
-    const uint32_t L = *large_used, T = *total_tiles;
+    const uint32_t T = *total_tiles;
     if (any_exec[pass] == 0)
     {
         if (status_next != nullptr)
@@ -409,7 +433,7 @@ __global__ __launch_bounds__(LARGE_BLOCK) void large_onesweep_kernel(
         {
             const uint32_t t = atomicAdd(queue, 1u);
             s_item = t;
-            if (t < T) s_j = find_segment(tile_start, L, t);
+            if (t < T) s_j = tile_seg[t];
         }
         for (uint32_t i = tid; i < W * RADIX; i += LARGE_BLOCK) s_hist[i] = 0;
         __syncthreads();
@@ -613,6 +637,8 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
     const std::size_t o_base = Lt.take(L_max * P * RADIX * 4);
     const std::size_t o_exec = Lt.take(L_max * P * 4);
     const std::size_t o_stB = Lt.take(T_max * RADIX * 4);
+    const std::size_t o_tmap = Lt.take(T_max * 4);
+    const std::size_t o_amap = Lt.take(T_max * 4);
     const std::size_t o_kalt = L_max ? Lt.take(num_items * sizeof(K)) : 0;
     const std::size_t o_valt = L_max && VALUES ? Lt.take(num_items * 4) : 0;
 
@@ -663,6 +689,8 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
     uint32_t* tstart = reinterpret_cast<uint32_t*>(base_ptr + o_tstart);
     uint32_t* active = reinterpret_cast<uint32_t*>(base_ptr + o_active);
     uint32_t* astart = reinterpret_cast<uint32_t*>(base_ptr + o_astart);
+    uint32_t* tmap = reinterpret_cast<uint32_t*>(base_ptr + o_tmap);
+    uint32_t* amap = reinterpret_cast<uint32_t*>(base_ptr + o_amap);
     uint32_t* hist = reinterpret_cast<uint32_t*>(base_ptr + o_hist);
     uint32_t* dbase = reinterpret_cast<uint32_t*>(base_ptr + o_base);
     uint32_t* st[2] = { reinterpret_cast<uint32_t*>(base_ptr + o_stA),
@@ -680,14 +708,19 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
         hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, ls, off, llist,
                            slot(LARGE), static_cast<uint32_t>(L_max),
                            static_cast<uint32_t>(T_max), TILE, tstart, large_used, total_tiles);
-        hipLaunchKernelGGL(hk, dim3(hgrid), dim3(256), 0, ls, keys, off, llist, tstart,
-                           large_used, total_tiles, hist);
+        const uint32_t mgrid = static_cast<uint32_t>(std::min<std::size_t>(L_max, 2048));
+        hipLaunchKernelGGL(tile_map_kernel, dim3(mgrid), dim3(256), 0, ls, tstart, large_used,
+                           total_tiles, tmap);
+        hipLaunchKernelGGL(hk, dim3(hgrid), dim3(256), 0, ls, keys, off, llist, tstart, tmap,
+                           total_tiles, hist);
         const uint32_t sgrid = static_cast<uint32_t>(std::min<std::size_t>(L_max * P, 4096));
         hipLaunchKernelGGL(large_digit_scan_kernel, dim3(sgrid), dim3(256), 0, ls, hist,
                            dbase, large_used, uint32_t(P), off, llist, exec, any_exec);
         hipLaunchKernelGGL(active_scan_kernel, dim3(1), dim3(1024), 0, ls, off, llist,
                            large_used, exec, uint32_t(P), static_cast<uint32_t>(T_max), TILE,
                            active, astart, act_used, act_tiles);
+        hipLaunchKernelGGL(tile_map_kernel, dim3(mgrid), dim3(256), 0, ls, astart, act_used,
+                           act_tiles, amap);
         K* kb[2] = { keys, reinterpret_cast<K*>(base_ptr + o_kalt) };
         uint32_t* vb[2] = { vals, VALUES ? reinterpret_cast<uint32_t*>(base_ptr + o_valt)
                                          : nullptr };
@@ -695,24 +728,28 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
         {
             timed_launch timer(ctx, "segmented_onesweep");
             hipLaunchKernelGGL(ok, dim3(ogrid), dim3(LARGE_BLOCK), olds, ls, kb[0], kb[1],
-                               vb[0], vb[1], off, llist, active, astart, act_used,
+                               vb[0], vb[1], off, llist, active, astart, amap,
                                act_tiles, uint32_t(p), dbase, exec, any_exec, st[p % 2],
                                p + 1 < P ? st[(p + 1) % 2] : nullptr, slot(24 + p));
         }
         hipLaunchKernelGGL(ck, dim3(cpgrid), dim3(256), 0, ls, kb[0], kb[1], vb[0], vb[1],
-                           off, llist, active, astart, act_used, act_tiles, exec,
+                           off, llist, active, astart, amap, act_tiles, exec,
                            uint32_t(P));
     };
 
     // The LDS classes sort disjoint segments: each is a branch forked from
     // stream s onto an auxiliary stream and joined back, so the classes'
-    // persistent kernels overlap (each drains its own queue, so a workgroup
-    // that starts after its queue is empty just exits) instead of running
-    // one after another with a drained GPU between them. Captured into a
-    // graph, the branches become parallel nodes. The large path follows on
-    // s once they are done: overlapped with them as well, it measured slower
-    // on the lognormal batches (2.17 against 2.05 ms; its look-back chains
-    // stall behind other kernels' workgroups).
+    // persistent kernels overlap (a workgroup whose share of its class is
+    // empty just exits) instead of running one after another with a
+    // drained GPU between them. Captured into a graph, the branches become
+    // parallel nodes. The classes fork from the largest down, the mid
+    // class last (forked first it measured slower: 7.04 against 6.61 ms for
+    // the learned sort of 2^28 lognormal keys, whose buckets fill the mid
+    // class and three others; the classes then compete for the CUs from
+    // the start). The large path follows on s once they are done:
+    // overlapped with them as well, it measured slower on the lognormal
+    // batches (2.17 against 2.05 ms; its look-back chains stall behind
+    // other kernels' workgroups).
     hipEvent_t fork = ctx.aux_event(0);
     CLC_HIP(hipEventRecord(fork, s));
     int branch = 0;
