@@ -330,7 +330,9 @@ clc_status clc_segmented_sort(clc_context ctx, void* keys, clc_type key_type,
 
 /* Times every workgroup configuration for each segment-size bucket
  * (bucket b: 2^(b-1) < size <= 2^b) on synthetic segments and keeps the
- * fastest, for this key type with or without values. The result is used by
+ * fastest, then for an LDS configuration the widest digit of its passes
+ * (8, 7, 6, 5 or 4 bits), for this key type with or without values. The
+ * result is used by
  * later calls on ctx and, when the context has a cache directory, written
  * there (one file per device, key type and values flag; entries from
  * another device or runtime are ignored and re-tuned). Synchronous. */
@@ -342,6 +344,13 @@ clc_status clc_segmented_sort_tune(clc_context ctx, clc_type key_type,
  * 1024x16 threads x keys), 7 = multi-workgroup onesweep. */
 clc_status clc_segmented_sort_get_table(clc_context ctx, clc_type key_type,
                                         int with_values, int* classes);
+
+/* The digit widths in use: bits[b] for the 32 buckets, 1..8, the widest
+ * digit of the LDS passes for segments of bucket b (keys that differ in
+ * their low k bits take ceil(k / bits[b]) passes, the k bits split evenly
+ * over them). No counterpart in the reference, which has no segmented sort. */
+clc_status clc_segmented_sort_get_digit_bits(clc_context ctx, clc_type key_type,
+                                             int with_values, int* bits);
 
 /* A segmented sort of fixed buffers captured once into a hipGraph, with its
  * own scratch; each clc_graph_launch replays the whole sort as one graph

@@ -120,6 +120,7 @@ SIGNATURES = {
     "clc_segmented_sort": (_i, [_vp, _vp, _i, _vp, _sz, _vp, _sz, _i]),
     "clc_segmented_sort_tune": (_i, [_vp, _i, _i]),
     "clc_segmented_sort_get_table": (_i, [_vp, _i, _i, ctypes.POINTER(ctypes.c_int)]),
+    "clc_segmented_sort_get_digit_bits": (_i, [_vp, _i, _i, ctypes.POINTER(ctypes.c_int)]),
     "clc_segmented_sort_graph_create": (_i, [_vp, _vp, _i, _vp, _sz, _vp, _sz, _i,
                                              ctypes.POINTER(_vp)]),
     "clc_learned_sort": (_i, [_vp, _vp, _i, _vp, _sz, _i, ctypes.POINTER(LearnedStats)]),
@@ -320,6 +321,12 @@ class Context:
         arr = (ctypes.c_int * 32)()
         check(lib().clc_segmented_sort_get_table(self.handle, key_type,
                                                  1 if with_values else 0, arr))
+        return list(arr)
+
+    def segmented_sort_digit_bits(self, key_type, with_values=True):
+        arr = (ctypes.c_int * 32)()
+        check(lib().clc_segmented_sort_get_digit_bits(self.handle, key_type,
+                                                      1 if with_values else 0, arr))
         return list(arr)
 
     def segmented_sort_graph(self, keys, values, offsets, descending=False, key_type=None):

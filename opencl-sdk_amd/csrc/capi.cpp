@@ -632,6 +632,17 @@ clc_status clc_segmented_sort_get_table(clc_context ctx, clc_type key_type,
     });
 }
 
+clc_status clc_segmented_sort_get_digit_bits(clc_context ctx, clc_type key_type,
+                                             int with_values, int* bits)
+{
+    return guarded([&] {
+        require(ctx != nullptr && bits != nullptr, "clc: null argument");
+        const std::vector<int> t =
+            clc::detail::segmented_sort_digit_bits(get(ctx), key_type, with_values != 0);
+        std::memcpy(bits, t.data(), t.size() * sizeof(int));
+    });
+}
+
 clc_status clc_segmented_sort_graph_create(clc_context ctx, void* keys,
                                            clc_type key_type, void* values,
                                            size_t num_items,

@@ -599,7 +599,7 @@ void run(context& ctx, uint32_t* keys, uint32_t* vals, std::size_t n, const uint
          uint32_t nseg, int key_type, batched_stats* out)
 {
     hipStream_t s = ctx.stream();
-    seg::table table;
+    seg::table table{};
     const std::vector<int> classes = detail::segmented_sort_table(ctx, key_type, VALUES);
     for (int b = 0; b < seg::NBUCKET; ++b) table.cls[b] = classes[b];
     const uint32_t cap = static_cast<uint32_t>(std::min<std::size_t>(lds_cap(table), 1u << 30));

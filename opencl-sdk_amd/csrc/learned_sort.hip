@@ -531,7 +531,7 @@ void run(context& ctx, K* keys, uint32_t* vals, uint32_t n, int key_type, stats*
     const uint32_t m = n / SAMPLE_STRIDE;
     uint32_t nb = 512;
     while (nb < 65536 && nb * 4096u < n) nb <<= 1;
-    seg::table table;
+    seg::table table{};
     const std::vector<int> classes = detail::segmented_sort_table(ctx, key_type, VALUES);
     for (int b = 0; b < seg::NBUCKET; ++b) table.cls[b] = classes[b];
     const std::size_t cap = lds_capacity(table);

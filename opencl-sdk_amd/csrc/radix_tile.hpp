@@ -450,7 +450,7 @@ __device__ __forceinline__ void digit_bases(uint32_t* s_hist, uint32_t* s_tmp)
 
 //! One stable LDS counting pass of a block over words w[] (row r < rows of
 //! wave `wave` holds position wave * 64 * rows + r * 64 + lane) by the
-//! nb-bit digit (w >> shift) & (2^nb - 1): wave-ballot ranks over nb bits,
+//! nb-bit digit (w >> shift) & (2^nb - 1): wave-ballot ranks,
 //! per-wave digit counters, an LDS reorder. Only positions below m are
 //! keys; rows of a wave that hold none are skipped, and the other positions
 //! past m take the largest digit, so they end behind every key (stability).
@@ -478,6 +478,9 @@ __device__ __forceinline__ void lds_pass(W (&w)[ITEMS], uint32_t (&v)[ITEMS], ui
             const bool valid = wave_base + r * WAVE + lane < m;
             const uint32_t d = valid ? static_cast<uint32_t>(w[r] >> shift) & mask : mask;
             dg[r / 4] = r % 4 == 0 ? d : dg[r / 4] | (d << (8 * (r % 4)));
+            // Eight ballots whatever nb is (the bits above nb are zero in
+            // every lane): skipping the unused ones took per-bit branches,
+            // and the compiler then spilled the ballot masks (4x slower).
             const uint64_t peers = match_digit(d);
             const uint32_t pre = count_below(peers);
             const uint32_t base = my_hist[d];
@@ -507,19 +510,19 @@ __device__ __forceinline__ void lds_pass(W (&w)[ITEMS], uint32_t (&v)[ITEMS], ui
 }
 
 //! Stable LSD passes of lds_pass over bits [lo, lo + span) of the words:
-//! ceil(span / 8) passes of 8 bits (the last one narrower).
+//! ceil(span / nbmax) passes of nbmax (1..8) bits, the last one narrower.
 template <typename W, int MOVE, int BLOCK, int ITEMS>
 __device__ __forceinline__ void lds_passes(W (&w)[ITEMS], uint32_t (&v)[ITEMS], uint32_t lo,
-                                           uint32_t span, uint32_t m, uint32_t rows,
-                                           uint32_t* s_hist, W* s_w, uint32_t* s_v,
-                                           uint32_t* s_tmp)
+                                           uint32_t span, uint32_t nbmax, uint32_t m,
+                                           uint32_t rows, uint32_t* s_hist, W* s_w,
+                                           uint32_t* s_v, uint32_t* s_tmp)
 {
-    const uint32_t passes = (span + 7) / 8;
+    const uint32_t passes = (span + nbmax - 1) / nbmax;
     uint32_t shift = lo;
 #pragma unroll 1
     for (uint32_t q = 0; q < passes; ++q)
     {
-        const uint32_t nb = min(8u, span - 8 * q);
+        const uint32_t nb = min(nbmax, span - nbmax * q);
         lds_pass<W, MOVE, BLOCK, ITEMS>(w, v, shift, nb, m, rows, s_hist, s_w, s_v, s_tmp);
         shift += nb;
     }
@@ -536,8 +539,9 @@ constexpr std::size_t lds_segment_bytes_per_key(std::size_t kbytes, bool values)
 //! Stable sort of m <= BLOCK * ITEMS keys (and values) by a single block,
 //! entirely in LDS, over s_buf (BLOCK * ITEMS * lds_segment_bytes_per_key
 //! bytes). The keys' order images are taken relative to their minimum:
-//! only ceil(log2(max - min + 1) / 8) digit places can differ, one pass
-//! each (none for one repeated key, which is left as it is). 64-bit keys
+//! only the low bits = ceil(log2(max - min + 1)) bits can differ, sorted in
+//! ceil(bits / nbmax) passes of at most nbmax (1..8) bits (none for one
+//! repeated key, which is left as it is). 64-bit keys
 //! with values are sorted as packed words, (digits << 16) | original
 //! position, 48 bits of the relative image at a time: the passes move 8
 //! bytes a key and no value, and each value is fetched once at the end by
@@ -546,7 +550,7 @@ constexpr std::size_t lds_segment_bytes_per_key(std::size_t kbytes, bool values)
 //! the caller's next barrier.
 template <typename K, int KIND, bool DESC, bool VALUES, int BLOCK, int ITEMS>
 __device__ __forceinline__ void lds_sort_segment(
-    K* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t m,
+    K* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t m, uint32_t nbmax,
     uint32_t* s_hist, void* s_buf, uint32_t* s_tmp)
 {
     constexpr bool WIDE = sizeof(K) == 8;
@@ -618,8 +622,12 @@ __device__ __forceinline__ void lds_sort_segment(
         __syncthreads(); // everyone has read s_min / s_max
         return;
     }
-    const uint32_t bits = WIDE ? 64u - __clzll(static_cast<unsigned long long>(range))
-                               : 32u - __clz(static_cast<uint32_t>(range));
+    // The same in every lane (read from LDS, which the compiler cannot
+    // know): made a scalar, so the pass loop is a plain loop rather than a
+    // lane-masked one.
+    const uint32_t bits = __builtin_amdgcn_readfirstlane(
+        WIDE ? 64u - __clzll(static_cast<unsigned long long>(range))
+             : 32u - __clz(static_cast<uint32_t>(range)));
 
     if constexpr (WIDE && VALUES)
     {
@@ -633,8 +641,8 @@ __device__ __forceinline__ void lds_sort_segment(
         for (int r = 0; r < ITEMS; ++r)
             k[r] = (static_cast<uint64_t>(k[r] - kmin) << 16) | (wave_base + r * WAVE + lane);
         const bool wide = bits > 48;
-        lds_passes<uint64_t, 0, BLOCK, ITEMS>(k, unused, 16, min(bits, 48u), m, rows, s_hist,
-                                              s_w, nullptr, s_tmp);
+        lds_passes<uint64_t, 0, BLOCK, ITEMS>(k, unused, 16, min(bits, 48u), nbmax, m, rows,
+                                              s_hist, s_w, nullptr, s_tmp);
         if (wide)
         {
             // Positions past m fetch position 0 (in bounds, never used).
@@ -654,8 +662,8 @@ __device__ __forceinline__ void lds_sort_segment(
                     const uint64_t rel = order_bits<K, KIND, DESC>(hi16[r]) - kmin;
                     k[r] = ((rel >> 48) << 16) | (k[r] & 0xFFFFu);
                 }
-            lds_passes<uint64_t, 0, BLOCK, ITEMS>(k, unused, 16, bits - 48, m, rows, s_hist,
-                                                  s_w, nullptr, s_tmp);
+            lds_passes<uint64_t, 0, BLOCK, ITEMS>(k, unused, 16, bits - 48, nbmax, m, rows,
+                                                  s_hist, s_w, nullptr, s_tmp);
         }
         // Each value (and, for a wide range, each key) by its original
         // position; all are read before any is overwritten in place (the
@@ -690,7 +698,7 @@ __device__ __forceinline__ void lds_sort_segment(
     uint32_t* s_v = reinterpret_cast<uint32_t*>(s_w + TILE);
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r) k[r] -= kmin;
-    lds_passes<K, MOVE, BLOCK, ITEMS>(k, v, 0, bits, m, rows, s_hist, s_w, s_v, s_tmp);
+    lds_passes<K, MOVE, BLOCK, ITEMS>(k, v, 0, bits, nbmax, m, rows, s_hist, s_w, s_v, s_tmp);
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r)
     {

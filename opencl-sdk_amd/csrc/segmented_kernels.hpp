@@ -23,7 +23,7 @@ __device__ __forceinline__ int bucket_of(uint64_t m)
 }
 
 //! Files every segment of >= 2 keys under its class: an LDS class gets the
-//! entry (begin low, begin high, size, 0) in its list (stride nseg), the
+//! entry (begin low, begin high, size, digit bits) in its list (stride nseg), the
 //! large class the segment's index. Segments that break the contract (end
 //! < begin, end > num_items, 2^30 keys or more) are left untouched, and the
 //! large class never takes more than large_cap segments, so malformed
@@ -41,6 +41,7 @@ __global__ __launch_bounds__(256) void classify_kernel(
     {
         const uint32_t i = base + threadIdx.x;
         int c = -1;
+        uint32_t nb = MAX_DIGIT_BITS;
         uint64_t b = 0, e = 0;
         if (i < nseg)
         {
@@ -50,6 +51,7 @@ __global__ __launch_bounds__(256) void classify_kernel(
             {
                 const int bk = bucket_of(e - b);
                 c = t.cls[bk];
+                nb = static_cast<uint32_t>(t.digit_bits[bk]);
                 if (use_mid && bk == MID_BUCKET && c < NCFG && e - b <= MID_CAP) c = MID;
             }
         }
@@ -71,7 +73,7 @@ __global__ __launch_bounds__(256) void classify_kernel(
             else // LDS classes; MID's list follows configuration NCFG - 1's
                 entries[size_t(k == MID ? NCFG : k) * nseg + pos] =
                     make_uint4(static_cast<uint32_t>(b), static_cast<uint32_t>(b >> 32),
-                               static_cast<uint32_t>(e - b), 0u);
+                               static_cast<uint32_t>(e - b), nb);
         }
     }
 }
@@ -98,8 +100,9 @@ __global__ __launch_bounds__(BLOCK) void lds_class_kernel(
     {
         const uint4 e = entries[i];
         const uint64_t b = e.x | (static_cast<uint64_t>(e.y) << 32);
+        const uint32_t nb = min(max(e.w, 1u), static_cast<uint32_t>(MAX_DIGIT_BITS));
         lds_sort_segment<K, KIND, DESC, VALUES, BLOCK, ITEMS>(
-            keys + b, VALUES ? vals + b : nullptr, e.z, s_hist, s_buf, s_tmp);
+            keys + b, VALUES ? vals + b : nullptr, e.z, nb, s_hist, s_buf, s_tmp);
         __syncthreads(); // the next segment reuses the LDS
     }
 }

@@ -22,6 +22,7 @@ table default_table(std::size_t kbytes, bool values, std::size_t lds_limit)
     for (int b = 0; b < NBUCKET; ++b)
     {
         t.cls[b] = LARGE;
+        t.digit_bits[b] = MAX_DIGIT_BITS;
         // Smallest LDS configuration that holds the whole bucket.
         for (int c = 0; c < NCFG; ++c)
         {
@@ -68,8 +69,30 @@ std::size_t lds_limit(const context& ctx)
 int tune_slot(int key_type, bool values) { return 1000 + key_type * 2 + (values ? 1 : 0); }
 
 // Tuned tables persist through the context's tuning cache (load_tuning /
-// store_tuning), one entry per key type and values flag; an entry that names
-// a class infeasible for this key type is treated as absent.
+// store_tuning), one entry per key type and values flag: the 32 classes,
+// then the 32 digit widths (an entry of the classes alone takes 8-bit
+// digits). An entry that names a class infeasible for this key type, or a
+// width outside 1..8, is treated as absent.
+
+std::vector<int> table_values(const table& t)
+{
+    std::vector<int> v(t.cls, t.cls + NBUCKET);
+    v.insert(v.end(), t.digit_bits, t.digit_bits + NBUCKET);
+    return v;
+}
+
+//! The table a value list holds (classes, then optionally digit widths),
+//! unchecked; false when the length fits neither form.
+bool table_from(const std::vector<int>& v, table& t)
+{
+    if (v.size() != std::size_t(NBUCKET) && v.size() != std::size_t(2 * NBUCKET)) return false;
+    for (int b = 0; b < NBUCKET; ++b)
+    {
+        t.cls[b] = v[b];
+        t.digit_bits[b] = v.size() == std::size_t(2 * NBUCKET) ? v[NBUCKET + b] : MAX_DIGIT_BITS;
+    }
+    return true;
+}
 
 std::string table_name(int key_type, bool values)
 {
@@ -79,13 +102,12 @@ std::string table_name(int key_type, bool values)
 bool load_table(const context& ctx, int key_type, bool values, table& t)
 {
     std::vector<int> v;
-    if (!load_tuning(ctx, table_name(key_type, values), v) || v.size() != NBUCKET)
-        return false;
     table r;
+    if (!load_tuning(ctx, table_name(key_type, values), v) || !table_from(v, r)) return false;
     for (int b = 0; b < NBUCKET; ++b)
     {
-        r.cls[b] = v[b];
         if (r.cls[b] < 0 || r.cls[b] > LARGE) return false;
+        if (r.digit_bits[b] < 1 || r.digit_bits[b] > MAX_DIGIT_BITS) return false;
         if (r.cls[b] < NCFG
             && (cfg_lds(r.cls[b], key_bytes(key_type), values) > lds_limit(ctx)
                 || (b > 0 && (std::size_t(1) << b) > cfg_capacity(r.cls[b]))))
@@ -97,8 +119,7 @@ bool load_table(const context& ctx, int key_type, bool values, table& t)
 
 void store_table(const context& ctx, int key_type, bool values, const table& t)
 {
-    store_tuning(ctx, table_name(key_type, values),
-                 std::vector<int>(t.cls, t.cls + NBUCKET));
+    store_tuning(ctx, table_name(key_type, values), table_values(t));
 }
 
 table table_for(context& ctx, int key_type, bool values)
@@ -107,14 +128,10 @@ table table_for(context& ctx, int key_type, bool values)
     const int slot = tune_slot(key_type, values);
     auto it = tuned.find(slot);
     table t;
-    if (it != tuned.end() && it->second.size() == NBUCKET)
-    {
-        for (int b = 0; b < NBUCKET; ++b) t.cls[b] = it->second[b];
-        return t;
-    }
+    if (it != tuned.end() && table_from(it->second, t)) return t;
     if (!load_table(ctx, key_type, values, t))
         t = default_table(key_bytes(key_type), values, lds_limit(ctx));
-    tuned[slot].assign(t.cls, t.cls + NBUCKET);
+    tuned[slot] = table_values(t);
     return t;
 }
 
@@ -134,7 +151,9 @@ __global__ void tune_fill_kernel(uint32_t* words, std::size_t n, uint64_t seed)
 }
 
 //! Times every feasible class for each size bucket on synthetic segments of
-//! that bucket and keeps the fastest.
+//! that bucket and keeps the fastest; for an LDS class then the widest
+//! digit of its passes, from 8 bits down to 4 (uniform 32-bit keys: a
+//! narrower digit means more passes and fewer ballots per pass).
 table run_tuner(context& ctx, int key_type, bool values)
 {
     const std::size_t kb = key_bytes(key_type);
@@ -171,16 +190,8 @@ table run_tuner(context& ctx, int key_type, bool values)
         for (std::size_t i = 0; i <= nseg; ++i) host_off[i] = i * m;
         CLC_HIP(hipMemcpyAsync(offs.get(), host_off.data(), (nseg + 1) * 8,
                                hipMemcpyHostToDevice, s));
-        std::vector<int> cands;
-        for (int c = 0; c < NCFG; ++c)
-            if (cfg_capacity(c) >= (std::size_t(1) << b) && cfg_lds(c, kb, values) <= limit)
-                cands.push_back(c);
-        if (b >= 12) cands.push_back(LARGE);
-        float best_ms = 1e30f;
-        for (int c : cands)
-        {
-            table t = best;
-            t.cls[b] = c;
+        // Best of three timed runs after one warm-up, on fresh keys each.
+        auto time_table = [&](const table& t) {
             float run_best = 1e30f;
             for (int rep = 0; rep < 4; ++rep)
             {
@@ -194,14 +205,41 @@ table run_tuner(context& ctx, int key_type, bool values)
                 CLC_HIP(hipEventSynchronize(e1));
                 float ms = 0;
                 CLC_HIP(hipEventElapsedTime(&ms, e0, e1));
-                if (rep > 0) run_best = std::min(run_best, ms); // rep 0 warms up
+                if (rep > 0) run_best = std::min(run_best, ms);
             }
-            if (run_best < best_ms)
+            return run_best;
+        };
+        std::vector<int> cands;
+        for (int c = 0; c < NCFG; ++c)
+            if (cfg_capacity(c) >= (std::size_t(1) << b) && cfg_lds(c, kb, values) <= limit)
+                cands.push_back(c);
+        if (b >= 12) cands.push_back(LARGE);
+        float best_ms = 1e30f;
+        for (int c : cands)
+        {
+            table t = best;
+            t.cls[b] = c;
+            t.digit_bits[b] = MAX_DIGIT_BITS;
+            const float ms = time_table(t);
+            if (ms < best_ms)
             {
-                best_ms = run_best;
+                best_ms = ms;
                 best.cls[b] = c;
+                best.digit_bits[b] = MAX_DIGIT_BITS;
             }
         }
+        if (best.cls[b] < NCFG)
+            for (int nb = MAX_DIGIT_BITS - 1; nb >= 4; --nb)
+            {
+                table t = best;
+                t.digit_bits[b] = nb;
+                const float ms = time_table(t);
+                if (ms < best_ms)
+                {
+                    best_ms = ms;
+                    best.digit_bits[b] = nb;
+                }
+            }
     }
     for (int b = MAX_BUCKET + 1; b < NBUCKET; ++b) best.cls[b] = LARGE;
     ctx.profiler().enable(profiling);
@@ -282,7 +320,7 @@ void segmented_sort_tune(context& ctx, int key_type, bool with_values)
     seg::pick(key_type);
     ctx.activate();
     const seg::table t = seg::run_tuner(ctx, key_type, with_values);
-    ctx.tuned()[seg::tune_slot(key_type, with_values)].assign(t.cls, t.cls + seg::NBUCKET);
+    ctx.tuned()[seg::tune_slot(key_type, with_values)] = seg::table_values(t);
     seg::store_table(ctx, key_type, with_values, t);
 }
 
@@ -292,6 +330,14 @@ std::vector<int> segmented_sort_table(context& ctx, int key_type, bool with_valu
     ctx.activate();
     const seg::table t = seg::table_for(ctx, key_type, with_values);
     return std::vector<int>(t.cls, t.cls + seg::NBUCKET);
+}
+
+std::vector<int> segmented_sort_digit_bits(context& ctx, int key_type, bool with_values)
+{
+    seg::pick(key_type);
+    ctx.activate();
+    const seg::table t = seg::table_for(ctx, key_type, with_values);
+    return std::vector<int>(t.digit_bits, t.digit_bits + seg::NBUCKET);
 }
 
 std::unique_ptr<seg::graph> segmented_sort_graph(context& ctx, void* keys, int key_type,

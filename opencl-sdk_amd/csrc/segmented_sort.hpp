@@ -84,7 +84,7 @@ inline std::size_t cfg_capacity(int c)
 }
 
 //! Dynamic LDS bytes of an LDS shape (block, capacity) for key size kbytes.
-inline std::size_t shape_lds(int block, std::size_t tile, std::size_t kbytes, bool values)
+constexpr std::size_t shape_lds(int block, std::size_t tile, std::size_t kbytes, bool values)
 {
     return std::size_t(block / 64) * 1024
         + tile * radix::lds_segment_bytes_per_key(kbytes, values) + 64;
@@ -103,10 +103,15 @@ inline std::size_t mid_lds(std::size_t kbytes, bool values)
 
 //! Class per bucket; the autotuner fills it, defaults come from
 //! default_table().
+//! Per bucket: its size class, and the widest digit (1..8 bits) of the
+//! LDS passes for its segments (a segment whose keys differ in `bits` low
+//! bits takes ceil(bits / digit_bits[b]) passes).
 struct table
 {
     int cls[NBUCKET];
+    int digit_bits[NBUCKET];
 };
+constexpr int MAX_DIGIT_BITS = 8;
 
 table default_table(std::size_t kbytes, bool values, std::size_t lds_limit);
 
@@ -146,6 +151,8 @@ void segmented_sort(context& ctx, void* keys, int key_type, void* values,
 void segmented_sort_tune(context& ctx, int key_type, bool with_values);
 //! The size-class table the segmented sort will use.
 std::vector<int> segmented_sort_table(context& ctx, int key_type, bool with_values);
+//! The widest digit (bits) of the LDS passes per segment-size bucket.
+std::vector<int> segmented_sort_digit_bits(context& ctx, int key_type, bool with_values);
 //! Captures the segmented sort of fixed buffers into a graph.
 std::unique_ptr<seg::graph> segmented_sort_graph(context& ctx, void* keys, int key_type,
                                                  void* values, std::size_t num_items,

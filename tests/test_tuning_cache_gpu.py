@@ -64,6 +64,7 @@ def test_segmented_table_round_trip_through_disk(tmp_path):
     try:
         a.segmented_sort_tune(clc_hip.U32, True)
         tuned = a.segmented_sort_table(clc_hip.U32, True)
+        tuned_bits = a.segmented_sort_digit_bits(clc_hip.U32, True)
     finally:
         a.close()
     files = glob.glob(os.path.join(cache, "clc_segsort_t5_kv_*.txt"))
@@ -71,13 +72,14 @@ def test_segmented_table_round_trip_through_disk(tmp_path):
     assert not glob.glob(os.path.join(cache, "*.tmp.*"))
     path = files[0]
     header, line, end = open(path).read().splitlines()
-    assert [int(x) for x in line.split()] == tuned
+    assert [int(x) for x in line.split()] == tuned + tuned_bits  # classes, then digit widths
     assert header.startswith("clc-tuning-v2 gfx950")
 
     say("tuning cache: a new context reads the stored table")
     b = clc_hip.Context(cache_dir=cache)
     try:
         assert b.segmented_sort_table(clc_hip.U32, True) == tuned
+        assert b.segmented_sort_digit_bits(clc_hip.U32, True) == tuned_bits
     finally:
         b.close()
 
