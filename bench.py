@@ -293,84 +293,124 @@ LEARNED_DISTS = {
 }
 
 
-def learned_leg(torch, clc_hip, ctx, log2n, steps, warmup):
-    n = 1 << log2n
+def learned_leg(torch, clc_hip, ctx, log2n, steps, warmup, small_log2n=24):
+    """The learned-bucket sort of uint64 keys + uint32 index values against
+    the plain radix sort, per skewed distribution, at 2^log2n and (the
+    grouping forms compared where 4096 buckets fit LDS) at 2^small_log2n:
+    the two-pass form (clc_learned_sort: two 8-bit id passes into up to
+    65536 buckets) and the single fused scatter into at most 4096 buckets
+    (clc_learned_sort_single_scatter)."""
     kb, vb = 8, 4
     # Algorithmic bytes of the learned form: inference + bucket histogram
     # read (k), one scatter into the buckets (2(k+v)), the LDS finish
-    # (2(k+v)): k + 4(k+v). Executed here: the id histogram (k), two id
-    # passes and the LDS finish (2(k+v) each): k + 6(k+v).
+    # (2(k+v)): k + 4(k+v). Executed by the two-pass form: the id histogram
+    # (k), two id passes and the LDS finish (2(k+v) each): k + 6(k+v). The
+    # single scatter executes the count (k), a copy of the input (2(k+v)),
+    # the scatter and the finish: also k + 6(k+v).
     lbytes = kb + 4 * (kb + vb)
     xbytes = kb + 6 * (kb + vb)
-    out = {"n": n, "key": "u64", "value": "u32 index",
+    out = {"n": 1 << log2n, "key": "u64", "value": "u32 index",
            "algorithmic_bytes_per_pair": {"learned": lbytes, "learned_executed": xbytes,
                                           "plain": kb + 8 * 2 * (kb + vb)}}
-    g = torch.Generator(device="cuda")
-    for i, (name, gen) in enumerate(LEARNED_DISTS.items()):
-        say("learned leg", name, f"2^{log2n}")
-        g.manual_seed(1000 + i)
+    grouping = ("learned_id_histogram", "learned_id_onesweep",
+                "learned_single_scatter_count", "learned_single_scatter")
+
+    def case(name, gen, n, seed, full):
+        g = torch.Generator(device="cuda")
+        g.manual_seed(seed)
         keys0 = gen(torch, n, g)
         vals0 = torch.arange(n, dtype=torch.int32, device="cuda")
-        kl, vl = torch.empty_like(keys0), torch.empty_like(vals0)
-        kp, vp = torch.empty_like(keys0), torch.empty_like(vals0)
+        bufs = {lab: (torch.empty_like(keys0), torch.empty_like(vals0))
+                for lab in ("learned", "single", "plain")}
 
-        def learned():
-            kl.copy_(keys0)
-            vl.copy_(vals0)
-            e0.record()
-            ctx.learned_sort(kl, vl, key_type=clc_hip.U64, stats=False)
-            e1.record()
-
-        def plain():
-            kp.copy_(keys0)
-            vp.copy_(vals0)
-            e0.record()
-            ctx.sort(kp, vp, key_type=clc_hip.U64)
-            e1.record()
+        def call(label, stats=False):
+            k, v = bufs[label]
+            if label == "plain":
+                ctx.sort(k, v, key_type=clc_hip.U64)
+                return None
+            return ctx.learned_sort(k, v, key_type=clc_hip.U64, stats=stats,
+                                    single_scatter=label == "single")
 
         res = {}
-        for label, fn in (("learned", learned), ("plain", plain)):
-            for _ in range(warmup):
-                e0 = torch.cuda.Event(enable_timing=True)
-                e1 = torch.cuda.Event(enable_timing=True)
-                fn()
+        for label in ("learned", "single", "plain"):
+            k, v = bufs[label]
             times = []
-            for _ in range(steps):
+            for it in range(warmup + steps):
+                k.copy_(keys0)
+                v.copy_(vals0)
                 e0 = torch.cuda.Event(enable_timing=True)
                 e1 = torch.cuda.Event(enable_timing=True)
-                fn()
+                e0.record()
+                call(label)
+                e1.record()
                 torch.cuda.synchronize()
-                times.append(e0.elapsed_time(e1))
+                if it >= warmup:
+                    times.append(e0.elapsed_time(e1))
             res[label] = statistics.fmean(times)
-        same = bool(torch.equal(kl, kp)) and bool(torch.equal(vl, vp))
-        # model statistics from one more call (reading them back adds a sync)
-        kl.copy_(keys0)
-        vl.copy_(vals0)
-        st = ctx.learned_sort(kl, vl, key_type=clc_hip.U64, stats=True)
-        torch.cuda.synchronize()
-        same = same and bool(torch.equal(kl, kp)) and bool(torch.equal(vl, vp))
-        distinct_top = int(torch.unique(keys0[: 1 << 20]).numel())
-        out[name] = {
+        kp, vp = bufs["plain"]
+        same = {lab: bool(torch.equal(bufs[lab][0], kp)) and bool(torch.equal(bufs[lab][1], vp))
+                for lab in ("learned", "single")}
+        # Kernel times of the grouping stage (library hipEvent profiler), and
+        # the model statistics, from one more call of each form.
+        ktimes, st = {}, {}
+        for label in ("learned", "single"):
+            k, v = bufs[label]
+            k.copy_(keys0)
+            v.copy_(vals0)
+            torch.cuda.synchronize()
+            ctx.reset_timings()
+            ctx.set_profiling(True)
+            st[label] = call(label, stats=True)
+            torch.cuda.synchronize()
+            t = ctx.kernel_timings()
+            ctx.set_profiling(False)
+            ktimes[label] = {kname: round(t[kname]["total_ms"], 4) for kname in grouping
+                             if kname in t}
+            same[label] = same[label] and bool(torch.equal(k, kp)) and bool(torch.equal(v, vp))
+        r = {
             "learned_ms": round(res["learned"], 4), "plain_ms": round(res["plain"], 4),
             "learned_gkeys": round(n / res["learned"] / 1e6, 3),
             "plain_gkeys": round(n / res["plain"] / 1e6, 3),
             "speedup": round(res["plain"] / res["learned"], 3),
-            "learned_roofline_frac": round(lbytes * n / (res["learned"] / 1e3)
-                                           / 1e9 / HBM_PEAK_GBS, 4),
-            "learned_executed_bytes_frac": round(xbytes * n / (res["learned"] / 1e3)
-                                                 / 1e9 / HBM_PEAK_GBS, 4),
-            "bit_identical_to_plain": same,
-            "used_model": st["used_model"], "num_buckets": st["num_buckets"],
-            "sample_size": st["sample_size"],
-            "sample_fit_error": round(st["sample_fit_error"], 5),
-            "max_bucket_overfill": round(st["max_bucket_overfill"], 3),
-            "fallback_fraction": round(st["fallback_fraction"], 5),
-            "root_log": st["root_log"],
-            "distinct_keys_in_first_2^20": distinct_top,
+            "bit_identical_to_plain": same["learned"],
+            "single_scatter_ms": round(res["single"], 4),
+            "single_scatter_speedup": round(res["plain"] / res["single"], 3),
+            "single_scatter_bit_identical_to_plain": same["single"],
+            "grouping_kernels_ms": ktimes,
+            "used_model": st["learned"]["used_model"],
+            "num_buckets": st["learned"]["num_buckets"],
+            "max_bucket_overfill": round(st["learned"]["max_bucket_overfill"], 3),
+            "fallback_fraction": round(st["learned"]["fallback_fraction"], 5),
+            "single_scatter_num_buckets": st["single"]["num_buckets"],
+            "single_scatter_max_bucket_overfill": round(st["single"]["max_bucket_overfill"], 3),
+            "single_scatter_fallback_fraction": round(st["single"]["fallback_fraction"], 5),
         }
-        say(name, out[name])
-        del keys0, vals0, kl, vl, kp, vp
+        if full:
+            r.update({
+                "learned_roofline_frac": round(lbytes * n / (res["learned"] / 1e3)
+                                               / 1e9 / HBM_PEAK_GBS, 4),
+                "learned_executed_bytes_frac": round(xbytes * n / (res["learned"] / 1e3)
+                                                     / 1e9 / HBM_PEAK_GBS, 4),
+                "sample_size": st["learned"]["sample_size"],
+                "sample_fit_error": round(st["learned"]["sample_fit_error"], 5),
+                "root_log": st["learned"]["root_log"],
+                "distinct_keys_in_first_2^20": int(torch.unique(keys0[: 1 << 20]).numel()),
+            })
+        del keys0, vals0, bufs
         torch.cuda.empty_cache()
+        return r
+
+    for i, (name, gen) in enumerate(LEARNED_DISTS.items()):
+        say("learned leg", name, f"2^{log2n}")
+        out[name] = case(name, gen, 1 << log2n, 1000 + i, True)
+        say(name, out[name])
+    if small_log2n:
+        small = {"n": 1 << small_log2n}
+        for i, (name, gen) in enumerate(LEARNED_DISTS.items()):
+            say("learned leg", name, f"2^{small_log2n}")
+            small[name] = case(name, gen, 1 << small_log2n, 2000 + i, False)
+            say(name, small[name])
+        out[f"at_2^{small_log2n}"] = small
     return out
 
 
@@ -823,8 +863,11 @@ def main() -> int:
     if "learned" in legs:
         lr = learned_leg(torch, clc_hip, ctx, args.learned_log2n, max(3, args.steps // 4),
                          1)
-        ok = ok and all(v["bit_identical_to_plain"] for k, v in lr.items()
-                        if isinstance(v, dict) and "bit_identical_to_plain" in v)
+        cases = [v for v in lr.values() if isinstance(v, dict) and "speedup" in v]
+        cases += [v for sub in lr.values() if isinstance(sub, dict)
+                  for v in sub.values() if isinstance(v, dict) and "speedup" in v]
+        ok = ok and all(v["bit_identical_to_plain"] and v["single_scatter_bit_identical_to_plain"]
+                        for v in cases)
         line["legs"]["learned"] = lr
     if "segmented" in legs:
         sg = segmented_leg(torch, clc_hip, ctx, max(3, args.steps // 2), 2,

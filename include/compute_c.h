@@ -403,6 +403,19 @@ clc_status clc_learned_sort(clc_context ctx, void* keys, clc_type key_type,
                             void* values, size_t n, int descending,
                             clc_learned_stats* stats);
 
+/* clc_learned_sort with the keys grouped by one fused scatter into at most
+ * 4096 buckets instead of two 8-bit passes: each workgroup counts the
+ * buckets of its contiguous chunk, one exclusive scan over the counts
+ * (bucket-major) fixes every bucket's size and each chunk's place in it,
+ * and each workgroup then writes its chunk's keys to their buckets tile by
+ * tile (a stable sort of the tile by bucket in LDS first). Same result;
+ * meant for inputs up to about 2^24 keys, where 4096 buckets fit LDS (a
+ * larger input leaves buckets above the LDS capacity to the onesweep
+ * fallback). */
+clc_status clc_learned_sort_single_scatter(clc_context ctx, void* keys, clc_type key_type,
+                                           void* values, size_t n, int descending,
+                                           clc_learned_stats* stats);
+
 /* Outcome of one clc_learned_segmented_sort call. */
 typedef struct clc_learned_batched_stats
 {

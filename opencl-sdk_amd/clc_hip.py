@@ -124,6 +124,8 @@ SIGNATURES = {
     "clc_segmented_sort_graph_create": (_i, [_vp, _vp, _i, _vp, _sz, _vp, _sz, _i,
                                              ctypes.POINTER(_vp)]),
     "clc_learned_sort": (_i, [_vp, _vp, _i, _vp, _sz, _i, ctypes.POINTER(LearnedStats)]),
+    "clc_learned_sort_single_scatter": (_i, [_vp, _vp, _i, _vp, _sz, _i,
+                                             ctypes.POINTER(LearnedStats)]),
     "clc_learned_segmented_sort": (_i, [_vp, _vp, _i, _vp, _sz, _vp, _sz, _i,
                                         ctypes.POINTER(LearnedBatchedStats)]),
     "clc_graph_launch": (_i, [_vp]),
@@ -279,12 +281,14 @@ class Context:
             ctypes.c_void_p(offsets.data_ptr()), ns, 1 if descending else 0))
 
     def learned_sort(self, keys, values=None, descending=False, key_type=None, n=None,
-                     stats=True):
-        """clc_learned_sort; returns the model statistics as a dict (or None)."""
+                     stats=True, single_scatter=False):
+        """clc_learned_sort (clc_learned_sort_single_scatter with
+        single_scatter); returns the model statistics as a dict (or None)."""
         kt = dtype_code(keys.dtype) if key_type is None else key_type
         n = keys.numel() if n is None else n
         st = LearnedStats() if stats else None
-        check(lib().clc_learned_sort(
+        fn = lib().clc_learned_sort_single_scatter if single_scatter else lib().clc_learned_sort
+        check(fn(
             self.handle, ctypes.c_void_p(keys.data_ptr()), kt,
             None if values is None else ctypes.c_void_p(values.data_ptr()), n,
             1 if descending else 0, ctypes.byref(st) if st is not None else None))

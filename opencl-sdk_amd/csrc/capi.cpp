@@ -672,14 +672,14 @@ clc_status clc_graph_launch(clc_graph graph)
 
 void clc_graph_release(clc_graph graph) { delete graph; }
 
-clc_status clc_learned_sort(clc_context ctx, void* keys, clc_type key_type,
-                            void* values, size_t n, int descending,
-                            clc_learned_stats* stats)
+namespace {
+clc_status learned_sort_call(clc_context ctx, void* keys, clc_type key_type, void* values,
+                             size_t n, int descending, bool single, clc_learned_stats* stats)
 {
     return guarded([&] {
         require(ctx != nullptr, "clc: null context");
         clc::learned::stats st;
-        clc::detail::learned_sort(get(ctx), keys, key_type, values, n, descending != 0,
+        clc::detail::learned_sort(get(ctx), keys, key_type, values, n, descending != 0, single,
                                   stats != nullptr ? &st : nullptr);
         if (stats != nullptr)
         {
@@ -692,6 +692,21 @@ clc_status clc_learned_sort(clc_context ctx, void* keys, clc_type key_type,
             stats->root_log = st.root_log;
         }
     });
+}
+} // namespace
+
+clc_status clc_learned_sort(clc_context ctx, void* keys, clc_type key_type,
+                            void* values, size_t n, int descending,
+                            clc_learned_stats* stats)
+{
+    return learned_sort_call(ctx, keys, key_type, values, n, descending, false, stats);
+}
+
+clc_status clc_learned_sort_single_scatter(clc_context ctx, void* keys, clc_type key_type,
+                                           void* values, size_t n, int descending,
+                                           clc_learned_stats* stats)
+{
+    return learned_sort_call(ctx, keys, key_type, values, n, descending, true, stats);
 }
 
 clc_status clc_learned_segmented_sort(clc_context ctx, void* keys, clc_type key_type,

@@ -30,6 +30,17 @@
 //                are sorted wholly in LDS; larger ones take its onesweep
 //                path (constant digit places skipped per bucket, so buckets
 //                of one repeated key cost nothing)
+//
+// The single-scatter form (clc_learned_sort_single_scatter) replaces the
+// histogram, the two passes and the boundaries with: per-workgroup bucket
+// counts of contiguous chunks (at most 4096 buckets), one exclusive scan of
+// the counts bucket-major (exact bucket sizes and each chunk's place in
+// every bucket), a copy of the input, and one stable scatter of each
+// chunk, tile by tile, sorted by bucket in LDS first. Measured slower than
+// the two passes on MI355X (bench learned leg): the scatter writes runs of
+// one or two keys per bucket per tile, and at 2^28 keys 4096 buckets hold
+// 64Ki keys each, beyond the LDS finish, so the fit check sends the input
+// to the plain sort.
 #include "context.hpp"
 #include "device_common.hpp"
 #include "learned_model.hpp"
@@ -504,6 +515,161 @@ __global__ __launch_bounds__(256) void bucket_stats_kernel(const uint64_t* __res
     }
 }
 
+// ---- the single fused scatter (at most 4096 buckets) ------------------------------
+
+//! Tile of the single scatter's workgroups, and its largest bucket count.
+constexpr int SBLOCK = 1024, SITEMS = 6;
+constexpr uint32_t STILE = SBLOCK * SITEMS;
+constexpr uint32_t SS_MAX_BUCKETS = 4096;
+
+//! Dynamic LDS of ss_scatter_kernel: per-wave digit counters, the running
+//! bucket offsets, each bucket's first sorted slot in the tile, the packed
+//! words, the tile's values and keys, and the model.
+constexpr std::size_t ss_lds(std::size_t kbytes, bool values)
+{
+    return std::size_t(SBLOCK / WAVE) * RADIX * 4 + 2 * SS_MAX_BUCKETS * 4 + STILE * 4
+        + (values ? STILE * 4 : 0) + STILE * kbytes + sizeof(rmi_model);
+}
+
+//! Workgroup w of the grid counts the buckets of its chunk [w * chunk,
+//! min(n, (w + 1) * chunk)) with the inference fused in, and stores them
+//! bucket-major, counts[b * grid + w]: one exclusive scan over the array
+//! then gives every (bucket, workgroup) pair its first output slot, so the
+//! bucket sizes and every workgroup's place inside each bucket are exact.
+template <typename K, int KIND, bool DESC>
+__global__ __launch_bounds__(1024) void ss_count_kernel(const K* __restrict__ keys, uint32_t n,
+                                                        uint32_t chunk, const rmi_model* model,
+                                                        uint32_t nb, uint32_t* __restrict__ counts)
+{
+    __shared__ rmi_model sm;
+    __shared__ uint32_t s_h[SS_MAX_BUCKETS];
+    const uint32_t tid = threadIdx.x;
+    load_model(&sm, model);
+    for (uint32_t b = tid; b < nb; b += 1024) s_h[b] = 0;
+    __syncthreads();
+    const uint32_t lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
+    constexpr uint32_t U = 4; // loads in flight per thread
+    for (uint32_t i0 = lo; i0 < hi; i0 += U * 1024)
+    {
+        K k[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) k[u] = keys[min(i0 + u * 1024 + tid, hi - 1)];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+            if (i0 + u * 1024 + tid < hi)
+            {
+                const uint32_t b = bucket_of(sm, static_cast<uint64_t>(order_bits<K, KIND, DESC>(k[u])));
+                atomicAdd(&s_h[min(b, nb - 1)], 1u);
+            }
+    }
+    __syncthreads();
+    for (uint32_t b = tid; b < nb; b += 1024)
+        counts[size_t(b) * gridDim.x + blockIdx.x] = s_h[b];
+}
+
+//! The single fused scatter: workgroup w walks its chunk in tiles of STILE
+//! keys, in order. Per tile: each key's bucket (inference), a stable sort
+//! of the words (bucket << 16 | position) in LDS by the bucket's id_bits
+//! bits (lds_passes), then each key goes to its bucket's running offset
+//! plus its rank among the tile's keys of that bucket, and the offsets
+//! advance by the tile's counts. Keys of one bucket keep their input order
+//! (tile order inside a workgroup, workgroup order from the scan), so the
+//! scatter is stable.
+template <typename K, int KIND, bool DESC, bool VALUES>
+__global__ __launch_bounds__(SBLOCK) void ss_scatter_kernel(
+    const K* __restrict__ keys_in, K* __restrict__ keys_out,
+    const uint32_t* __restrict__ vals_in, uint32_t* __restrict__ vals_out, uint32_t n,
+    uint32_t chunk, const rmi_model* model, uint32_t nb, uint32_t id_bits,
+    const uint32_t* __restrict__ offsets)
+{
+    constexpr int W = SBLOCK / WAVE;
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint32_t* s_hist = reinterpret_cast<uint32_t*>(smem);
+    uint32_t* s_off = s_hist + W * RADIX;
+    uint32_t* s_first = s_off + SS_MAX_BUCKETS;
+    uint32_t* s_w = s_first + SS_MAX_BUCKETS;
+    uint32_t* s_vals = s_w + STILE;
+    K* s_keys = reinterpret_cast<K*>(s_vals + (VALUES ? STILE : 0));
+    rmi_model* s_model = reinterpret_cast<rmi_model*>(s_keys + STILE);
+    __shared__ uint32_t s_tmp[4];
+    const uint32_t tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
+    for (uint32_t i = tid; i < W * RADIX; i += SBLOCK) s_hist[i] = 0;
+    for (uint32_t b = tid; b < nb; b += SBLOCK)
+        s_off[b] = offsets[size_t(b) * gridDim.x + blockIdx.x];
+    load_model(s_model, model);
+    __syncthreads();
+    const uint32_t lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
+    // Position of row r of this thread (the layout lds_pass expects).
+    const uint32_t row0 = wave * (WAVE * SITEMS) + lane;
+    for (uint32_t t0 = lo; t0 < hi; t0 += STILE)
+    {
+        const uint32_t m = min(STILE, hi - t0);
+        uint32_t w[SITEMS], unused[SITEMS];
+#pragma unroll
+        for (int r = 0; r < SITEMS; ++r)
+        {
+            const uint32_t p = row0 + r * WAVE;
+            w[r] = 0;
+            if (p < m)
+            {
+                const K k = keys_in[t0 + p];
+                s_keys[p] = k;
+                if constexpr (VALUES) s_vals[p] = vals_in[t0 + p];
+                const uint32_t b =
+                    bucket_of(*s_model, static_cast<uint64_t>(order_bits<K, KIND, DESC>(k)));
+                w[r] = (min(b, nb - 1) << 16) | p;
+            }
+        }
+        lds_passes<uint32_t, 0, SBLOCK, SITEMS>(w, unused, 16, id_bits, 8, m, SITEMS, s_hist,
+                                                s_w, nullptr, s_tmp);
+        // s_w now holds the tile's words in sorted order (slots below m),
+        // and w[r] the word at slot row0 + r * 64. First slot of each run.
+#pragma unroll
+        for (int r = 0; r < SITEMS; ++r)
+        {
+            const uint32_t i = row0 + r * WAVE;
+            if (i < m)
+            {
+                const uint32_t id = w[r] >> 16;
+                if (i == 0 || (s_w[i - 1] >> 16) != id) s_first[id] = i;
+            }
+        }
+        __syncthreads();
+        uint32_t run_len[SITEMS];
+#pragma unroll
+        for (int r = 0; r < SITEMS; ++r)
+        {
+            const uint32_t i = row0 + r * WAVE;
+            run_len[r] = 0;
+            if (i < m)
+            {
+                const uint32_t id = w[r] >> 16, pos = w[r] & 0xFFFFu;
+                const uint32_t first = s_first[id];
+                const uint32_t dest = s_off[id] + (i - first);
+                keys_out[dest] = s_keys[pos];
+                if constexpr (VALUES) vals_out[dest] = s_vals[pos];
+                if (i + 1 == m || (s_w[i + 1] >> 16) != id) run_len[r] = i + 1 - first;
+            }
+        }
+        __syncthreads(); // every offset read before any advances
+#pragma unroll
+        for (int r = 0; r < SITEMS; ++r)
+            if (run_len[r] != 0) s_off[w[r] >> 16] += run_len[r];
+        __syncthreads(); // offsets advanced; the tile's LDS is free again
+    }
+}
+
+//! Bucket offsets of the single scatter: off[b] = first slot of bucket b
+//! (its first workgroup's offset), off[nb] = n.
+__global__ __launch_bounds__(256) void ss_bounds_kernel(const uint32_t* __restrict__ offsets,
+                                                        uint32_t groups, uint32_t nb, uint32_t n,
+                                                        uint64_t* __restrict__ off)
+{
+    const uint32_t b = blockIdx.x * 256 + threadIdx.x;
+    if (b < nb) off[b] = offsets[size_t(b) * groups];
+    else if (b == nb) off[b] = n;
+}
+
 struct layout
 {
     std::size_t bytes = 0;
@@ -525,12 +691,16 @@ std::size_t lds_capacity(const seg::table& t)
 }
 
 template <typename K, int KIND, bool DESC, bool VALUES>
-void run(context& ctx, K* keys, uint32_t* vals, uint32_t n, int key_type, stats* out)
+void run(context& ctx, K* keys, uint32_t* vals, uint32_t n, int key_type, bool single,
+         stats* out)
 {
     hipStream_t s = ctx.stream();
     const uint32_t m = n / SAMPLE_STRIDE;
+    // Buckets: about 4096 keys each, 512 to 65536 (two 8-bit passes), or at
+    // most 4096 for the single scatter.
+    const uint32_t nb_max = single ? SS_MAX_BUCKETS : 65536u;
     uint32_t nb = 512;
-    while (nb < 65536 && nb * 4096u < n) nb <<= 1;
+    while (nb < nb_max && nb * 4096u < n) nb <<= 1;
     seg::table table{};
     const std::vector<int> classes = detail::segmented_sort_table(ctx, key_type, VALUES);
     for (int b = 0; b < seg::NBUCKET; ++b) table.cls[b] = classes[b];
@@ -639,38 +809,85 @@ void run(context& ctx, K* keys, uint32_t* vals, uint32_t n, int key_type, stats*
         return;
     }
 
+    if (single)
     {
-        timed_launch timer(ctx, "learned_id_histogram");
-        hipLaunchKernelGGL((id_histogram_kernel<K, KIND, DESC>), dim3(grid), dim3(256), 0, s,
-                           keys, n, model, hist);
+        // Workgroups of one resident tile each (the scatter's LDS holds one
+        // per CU), chunks of whole tiles.
+        const uint32_t groups_want = std::max(1u, uint32_t(ctx.facts().cu_count));
+        const uint32_t chunk =
+            std::max(1u, (n + groups_want * STILE - 1) / (groups_want * STILE)) * STILE;
+        const uint32_t groups = (n + chunk - 1) / chunk;
+        const std::size_t cells = std::size_t(nb) * groups;
+        temp_buffer cnt(ctx.resource(), 2 * cells * 4, ctx.stream());
+        uint32_t* counts = cnt.as<uint32_t>();
+        uint32_t* offsets = counts + cells;
+        uint32_t id_bits = 0;
+        while ((1u << id_bits) < nb) ++id_bits;
+        {
+            timed_launch timer(ctx, "learned_single_scatter_count");
+            hipLaunchKernelGGL((ss_count_kernel<K, KIND, DESC>), dim3(groups), dim3(1024), 0, s,
+                               keys, n, chunk, model, nb, counts);
+        }
+        const uint32_t zero = 0;
+        detail::scan(ctx, counts, offsets, cells, TY_U32, OP_PLUS, &zero, true);
+        // The scatter reads the keys from a copy and writes them to their
+        // buckets in place of the input.
+        CLC_HIP(hipMemcpyAsync(kalt, keys, std::size_t(n) * sizeof(K), hipMemcpyDeviceToDevice, s));
+        if (VALUES)
+            CLC_HIP(hipMemcpyAsync(valt, vals, std::size_t(n) * 4, hipMemcpyDeviceToDevice, s));
+        const std::size_t slds = ss_lds(sizeof(K), VALUES);
+        auto sk = ss_scatter_kernel<K, KIND, DESC, VALUES>;
+        static bool s_opted = false;
+        if (!s_opted)
+        {
+            CLC_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(sk),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        static_cast<int>(slds)));
+            s_opted = true;
+        }
+        {
+            timed_launch timer(ctx, "learned_single_scatter");
+            hipLaunchKernelGGL(sk, dim3(groups), dim3(SBLOCK), slds, s, kalt, keys, valt, vals, n,
+                               chunk, model, nb, id_bits, offsets);
+        }
+        hipLaunchKernelGGL(ss_bounds_kernel, dim3(nb / 256 + 1), dim3(256), 0, s, offsets,
+                           groups, nb, n, off);
     }
-    hipLaunchKernelGGL(id_scan_kernel, dim3(2), dim3(256), 0, s, hist, base);
-    const std::size_t lds = std::size_t(LBLOCK / WAVE) * 1024 + RADIX * 4
-        + ((sizeof(rmi_model) + 15) & ~std::size_t(15)) + std::size_t(LTILE) * sizeof(K)
-        + std::size_t(LTILE) + 64;
-    auto kernel = id_onesweep_kernel<K, KIND, DESC, VALUES>;
-    static bool opted = false;
-    if (!opted && lds > 65536)
+    else
     {
-        CLC_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    static_cast<int>(lds)));
-        opted = true;
-    }
-    {
-        timed_launch timer(ctx, "learned_id_onesweep");
-        hipLaunchKernelGGL(kernel, dim3(tiles), dim3(LBLOCK), lds, s, keys, kalt, vals, valt, n,
-                           0u, model, base, st[0], st[1], ctr);
-    }
-    {
-        timed_launch timer(ctx, "learned_id_onesweep");
-        hipLaunchKernelGGL(kernel, dim3(tiles), dim3(LBLOCK), lds, s, kalt, keys, valt, vals, n,
-                           8u, model, base + RADIX, st[1], nullptr, ctr + 1);
-    }
-    {
-        timed_launch timer(ctx, "learned_boundaries");
-        hipLaunchKernelGGL((boundary_kernel<K, KIND, DESC>), dim3(nb / 256 + 1), dim3(256), 0,
-                           s, keys, n, model, nb, off);
+        {
+            timed_launch timer(ctx, "learned_id_histogram");
+            hipLaunchKernelGGL((id_histogram_kernel<K, KIND, DESC>), dim3(grid), dim3(256), 0, s,
+                               keys, n, model, hist);
+        }
+        hipLaunchKernelGGL(id_scan_kernel, dim3(2), dim3(256), 0, s, hist, base);
+        const std::size_t lds = std::size_t(LBLOCK / WAVE) * 1024 + RADIX * 4
+            + ((sizeof(rmi_model) + 15) & ~std::size_t(15)) + std::size_t(LTILE) * sizeof(K)
+            + std::size_t(LTILE) + 64;
+        auto kernel = id_onesweep_kernel<K, KIND, DESC, VALUES>;
+        static bool opted = false;
+        if (!opted && lds > 65536)
+        {
+            CLC_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        static_cast<int>(lds)));
+            opted = true;
+        }
+        {
+            timed_launch timer(ctx, "learned_id_onesweep");
+            hipLaunchKernelGGL(kernel, dim3(tiles), dim3(LBLOCK), lds, s, keys, kalt, vals, valt, n,
+                               0u, model, base, st[0], st[1], ctr);
+        }
+        {
+            timed_launch timer(ctx, "learned_id_onesweep");
+            hipLaunchKernelGGL(kernel, dim3(tiles), dim3(LBLOCK), lds, s, kalt, keys, valt, vals, n,
+                               8u, model, base + RADIX, st[1], nullptr, ctr + 1);
+        }
+        {
+            timed_launch timer(ctx, "learned_boundaries");
+            hipLaunchKernelGGL((boundary_kernel<K, KIND, DESC>), dim3(nb / 256 + 1), dim3(256), 0,
+                               s, keys, n, model, nb, off);
+        }
     }
     detail::segmented_sort(ctx, keys, key_type, vals, n, off, nb, DESC);
     if (out != nullptr)
@@ -691,19 +908,19 @@ void run(context& ctx, K* keys, uint32_t* vals, uint32_t n, int key_type, stats*
 
 template <typename K, int KIND>
 void run_typed(context& ctx, void* keys, void* values, uint32_t n, bool desc, int key_type,
-               stats* out)
+               bool single, stats* out)
 {
     K* k = static_cast<K*>(keys);
     uint32_t* v = static_cast<uint32_t*>(values);
     if (desc)
     {
-        if (v) run<K, KIND, true, true>(ctx, k, v, n, key_type, out);
-        else run<K, KIND, true, false>(ctx, k, v, n, key_type, out);
+        if (v) run<K, KIND, true, true>(ctx, k, v, n, key_type, single, out);
+        else run<K, KIND, true, false>(ctx, k, v, n, key_type, single, out);
     }
     else
     {
-        if (v) run<K, KIND, false, true>(ctx, k, v, n, key_type, out);
-        else run<K, KIND, false, false>(ctx, k, v, n, key_type, out);
+        if (v) run<K, KIND, false, true>(ctx, k, v, n, key_type, single, out);
+        else run<K, KIND, false, false>(ctx, k, v, n, key_type, single, out);
     }
 }
 
@@ -713,7 +930,7 @@ void run_typed(context& ctx, void* keys, void* values, uint32_t n, bool desc, in
 namespace detail {
 
 void learned_sort(context& ctx, void* keys, int key_type, void* values, std::size_t n,
-                  bool descending, learned::stats* out)
+                  bool descending, bool single_scatter, learned::stats* out)
 {
     switch (key_type)
     {
@@ -739,12 +956,12 @@ void learned_sort(context& ctx, void* keys, int key_type, void* values, std::siz
     using namespace learned;
     switch (key_type)
     {
-        case TY_U32: run_typed<uint32_t, KEY_UNSIGNED>(ctx, keys, values, m, descending, key_type, out); break;
-        case TY_I32: run_typed<uint32_t, KEY_SIGNED>(ctx, keys, values, m, descending, key_type, out); break;
-        case TY_F32: run_typed<uint32_t, KEY_FLOAT>(ctx, keys, values, m, descending, key_type, out); break;
-        case TY_U64: run_typed<uint64_t, KEY_UNSIGNED>(ctx, keys, values, m, descending, key_type, out); break;
-        case TY_I64: run_typed<uint64_t, KEY_SIGNED>(ctx, keys, values, m, descending, key_type, out); break;
-        case TY_F64: run_typed<uint64_t, KEY_FLOAT>(ctx, keys, values, m, descending, key_type, out); break;
+        case TY_U32: run_typed<uint32_t, KEY_UNSIGNED>(ctx, keys, values, m, descending, key_type, single_scatter, out); break;
+        case TY_I32: run_typed<uint32_t, KEY_SIGNED>(ctx, keys, values, m, descending, key_type, single_scatter, out); break;
+        case TY_F32: run_typed<uint32_t, KEY_FLOAT>(ctx, keys, values, m, descending, key_type, single_scatter, out); break;
+        case TY_U64: run_typed<uint64_t, KEY_UNSIGNED>(ctx, keys, values, m, descending, key_type, single_scatter, out); break;
+        case TY_I64: run_typed<uint64_t, KEY_SIGNED>(ctx, keys, values, m, descending, key_type, single_scatter, out); break;
+        case TY_F64: run_typed<uint64_t, KEY_FLOAT>(ctx, keys, values, m, descending, key_type, single_scatter, out); break;
     }
 }
 

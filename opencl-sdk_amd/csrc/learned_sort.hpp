@@ -39,9 +39,11 @@ struct batched_stats
 namespace detail {
 
 //! Stable sort (same result as radix_sort) through the learned buckets;
-//! fills out when given (which costs one extra synchronisation).
+//! fills out when given (which costs one extra synchronisation). With
+//! single_scatter, keys reach at most 4096 buckets in one fused scatter
+//! instead of two 8-bit passes into up to 65536.
 void learned_sort(context& ctx, void* keys, int key_type, void* values, std::size_t n,
-                  bool descending, learned::stats* out);
+                  bool descending, bool single_scatter, learned::stats* out);
 
 //! Stable sort of every segment [offsets[i], offsets[i+1]) of 32-bit keys
 //! (and values) with one learned model per segment (learned_batched.hip);
