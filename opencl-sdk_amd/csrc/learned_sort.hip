@@ -6,10 +6,11 @@
 //                sample, sorted by the plain radix sort
 //   fit          a two-level RMI by reduction kernels: one root linear model
 //                mapping the key to one of 256 leaves, and per leaf a linear
-//                model of the empirical CDF. Two candidates are fitted at
-//                once, the root linear in the key or in its log image (keys
-//                spread over many octaves), and the one whose sample
-//                predicts less overfill is kept. Slopes are clamped >= 0 and each
+//                model of the empirical CDF, both over the same feature.
+//                Two candidates are fitted at once, the feature being the
+//                key or its log image (keys spread over many octaves), and
+//                the one whose sample predicts less overfill is kept.
+//                Slopes are clamped >= 0 and each
 //                leaf's bucket range ends where the next leaf's begins, so
 //                key -> bucket is a monotone non-decreasing pure function
 //                (conversion to double and IEEE rounding are monotone too):
@@ -228,7 +229,8 @@ __global__ __launch_bounds__(256) void leaf_units_kernel(const uint32_t* __restr
     if (l == LEAVES - 1) ustart[LEAVES] = before + u;
 }
 
-//! Leaf moments: one unit of one leaf, relative to the leaf's first key and
+//! Leaf moments: one unit of one leaf, in the candidate's root feature (the
+//! key or its log image), relative to the leaf's first key's feature and
 //! index. Launched with an upper bound of units (units_max per candidate);
 //! extra blocks return.
 __global__ __launch_bounds__(256) void leaf_partial_kernel(const uint64_t* __restrict__ s,
@@ -251,16 +253,18 @@ __global__ __launch_bounds__(256) void leaf_partial_kernel(const uint64_t* __res
     const uint32_t l = lo_l;
     const uint32_t first = lstart[l];
     const uint32_t lo = first + (b - ustart[l]) * FIT_UNIT;
+    const bool log = blockIdx.y == 1;
     double mo[NMOM];
-    unit_moments(s, lo, min(lstart[l + 1], lo + FIT_UNIT), s[first], first, false, mo);
+    unit_moments(s, lo, min(lstart[l + 1], lo + FIT_UNIT), log ? log_image(s[first]) : s[first],
+                 first, log, mo);
     if (threadIdx.x == 0)
         for (int k = 0; k < NMOM; ++k)
             part[(size_t(blockIdx.y) * units_max + b) * NMOM + k] = mo[k];
 }
 
-//! Leaf l: linear model of the empirical CDF (slope clamped >= 0) and its
-//! bucket range [lo, hi] (proportional to its share of the sample; each
-//! range ends where the next begins).
+//! Leaf l: linear model of the empirical CDF in the root feature (slope
+//! clamped >= 0) and its bucket range [lo, hi] (proportional to its share
+//! of the sample; each range ends where the next begins).
 __global__ __launch_bounds__(256) void leaf_fit_kernel(const uint64_t* __restrict__ s,
                                                        uint32_t m, uint32_t nb,
                                                        const uint32_t* __restrict__ lstart_all,
@@ -282,9 +286,10 @@ __global__ __launch_bounds__(256) void leaf_fit_kernel(const uint64_t* __restric
         if (hi - lo >= 2)
         {
             double ai, ci;
-            line_from_moments(mo, ai, ci); // index - lo ~ ai * (x - s[lo]) + ci
+            line_from_moments(mo, ai, ci); // index - lo ~ ai * (f - f(s[lo])) + ci
+            const uint64_t fref = c == 1 ? log_image(s[lo]) : s[lo];
             a = ai / double(m);
-            b = (double(lo) + ci - ai * static_cast<double>(s[lo])) / double(m);
+            b = (double(lo) + ci - ai * static_cast<double>(fref)) / double(m);
         }
         const uint32_t blo = static_cast<uint32_t>(uint64_t(nb) * lo / m);
         const uint32_t next =
