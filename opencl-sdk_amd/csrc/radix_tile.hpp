@@ -580,7 +580,7 @@ __device__ __forceinline__ void lds_sort_segment(
     // (row tests here also cost the compiler registers).
     // 32-bit keys load their values with them (one round trip); 64-bit
     // keys fetch values once, by position, after sorting.
-    constexpr bool EARLY_VALUES = VALUES && !WIDE;
+    constexpr bool EARLY_VALUES = false;
     K k[ITEMS];
     uint32_t v[ITEMS];
 #pragma unroll
@@ -693,7 +693,45 @@ __device__ __forceinline__ void lds_sort_segment(
         return; // no LDS use after the last pass's barrier
 
     }
-    constexpr int MOVE = VALUES ? 1 : 0; // 32-bit keys: values beside them
+    if constexpr (!WIDE && VALUES)
+    {
+        // 32-bit keys with values: 64-bit words (relative key << 16) |
+        // position, as for 64-bit keys (the relative key has at most 32
+        // bits): one 8-byte LDS write and read a key and pass instead of
+        // two 4-byte ones for key and value, and the values, unneeded by
+        // the passes, fetched once by position at the end.
+        uint64_t* s_w = static_cast<uint64_t*>(s_buf);
+        uint32_t unused[ITEMS];
+        uint64_t w[ITEMS];
+#pragma unroll
+        for (int r = 0; r < ITEMS; ++r)
+            w[r] = (static_cast<uint64_t>(k[r] - kmin) << 16) | (wave_base + r * WAVE + lane);
+        lds_passes<uint64_t, 0, BLOCK, ITEMS>(w, unused, 16, bits, nbmax, m, rows, s_hist, s_w,
+                                              nullptr, s_tmp);
+#pragma unroll
+        for (int r = 0; r < ITEMS; ++r)
+            if (static_cast<uint32_t>(r) < live)
+            {
+                const uint32_t at = wave_base + r * WAVE + lane < m
+                    ? static_cast<uint32_t>(w[r]) & 0xFFFFu : 0u;
+                v[r] = vals[at];
+                k[r] = from_order_bits<K, KIND, DESC>(static_cast<K>(kmin + (w[r] >> 16)));
+            }
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads(); // every value read before any is overwritten in place
+#pragma unroll
+        for (int r = 0; r < ITEMS; ++r)
+        {
+            const uint32_t pos = wave_base + r * WAVE + lane;
+            if (static_cast<uint32_t>(r) < rows && pos < m)
+            {
+                keys[pos] = k[r];
+                vals[pos] = v[r];
+            }
+        }
+        return;
+    }
+    constexpr int MOVE = 0; // 32-bit keys without values
     K* s_w = static_cast<K*>(s_buf);
     uint32_t* s_v = reinterpret_cast<uint32_t*>(s_w + TILE);
 #pragma unroll
