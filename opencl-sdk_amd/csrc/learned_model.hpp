@@ -1,8 +1,7 @@
 // The learned CDF model shared by the learned-bucket sorts (single array
 // and batched segments): a two-level RMI, one root linear model over a
 // feature of the key choosing one of 256 leaves, one linear CDF model per
-// leaf over the same feature, and the monotone key -> bucket map it
-// defines.
+// leaf, and the monotone key -> bucket map it defines.
 #pragma once
 
 #include "device_common.hpp"
@@ -19,7 +18,7 @@ constexpr uint32_t SAMPLE_STRIDE = 100;
 
 struct rmi_leaf
 {
-    double a, b; // bucket position ~ a * f + b, f the root feature (the CDF times the bucket count)
+    double a, b; // bucket position ~ a * x + b (the CDF times the bucket count)
     uint32_t lo, hi; // bucket range of the leaf
 };
 
@@ -54,21 +53,17 @@ __device__ __forceinline__ uint32_t leaf_of(const rmi_model& m, double f)
     return static_cast<uint32_t>(r);
 }
 
-//! Monotone non-decreasing bucket of an order image: the root feature f,
+//! Monotone non-decreasing bucket of an order image x: the root feature,
 //! its conversion to double and every model step are monotone, leaf slopes
-//! are >= 0 and leaf bucket ranges are ordered. The leaf's line is in f as
-//! well (for the log root, a tail leaf spanning several octaves of a
-//! lognormal is far closer to linear in the log image than in the key: on
-//! 2^24 lognormal(20, 2) keys its overfull buckets held 3.1% of the keys
-//! with lines in the key and 0.5% with lines in f). The position a * f + b
-//! is truncated (clamped to the u32 range first) and clamped to the leaf's
+//! are >= 0 and leaf bucket ranges are ordered. The position a * x + b is
+//! truncated (clamped to the u32 range first) and clamped to the leaf's
 //! [lo, hi] in integers, which equals clamping it in doubles first: for
 //! integers lo <= hi, trunc(p) <= lo when p <= lo and >= hi when p >= hi.
 __device__ __forceinline__ uint32_t bucket_of(const rmi_model& m, uint64_t bits)
 {
-    const double f = root_feature(m, bits);
-    const rmi_leaf& L = m.leaf[leaf_of(m, f)];
-    const double p = fmin(fmax(fma(L.a, f, L.b), 0.0), 4294967295.0);
+    const double x = static_cast<double>(bits);
+    const rmi_leaf& L = m.leaf[leaf_of(m, root_feature(m, bits))];
+    const double p = fmin(fmax(fma(L.a, x, L.b), 0.0), 4294967295.0);
     return min(max(static_cast<uint32_t>(p), L.lo), L.hi);
 }
 

@@ -4,11 +4,12 @@
 //
 //   sample       every 100th key (its order image, as u64): a strided 1%
 //                sample, sorted by the plain radix sort
-//   fit          a two-level RMI by reduction kernels: one root linear model
-//                mapping the key to one of 256 leaves, and per leaf a linear
-//                model of the empirical CDF, both over the same feature.
-//                Two candidates are fitted at once, the feature being the
-//                key or its log image (keys spread over many octaves), and
+//   fit          a two-level RMI: one root linear model mapping the key to
+//                one of 256 leaves (the secant of the sample's CDF between
+//                two quantiles), and per leaf a linear model of the
+//                empirical CDF (least squares by reduction kernels). Two
+//                candidates are fitted at once, the root linear in the key
+//                or in its log image (keys spread over many octaves), and
 //                the one whose sample predicts less overfill is kept.
 //                Slopes are clamped >= 0 and each
 //                leaf's bucket range ends where the next leaf's begins, so
@@ -103,19 +104,20 @@ __global__ __launch_bounds__(256) void sample_kernel(const K* __restrict__ keys,
 constexpr uint32_t FIT_UNIT = 4096;
 constexpr int NMOM = 5; // count, sum x, sum y, sum x^2, sum x*y
 constexpr uint32_t CANDIDATES = 2;
+//! The root's secant runs through the sample's quantiles 1/ROOT_QUANTILE
+//! and 1 - 1/ROOT_QUANTILE.
+constexpr uint32_t ROOT_QUANTILE = 512;
 
-//! Moments of samples [lo, hi) of the 256-thread block, reduced in a fixed
-//! order into out[] (thread 0). With log set, x is the log image of the key
-//! (xref is then a log image too).
+//! Moments of samples [lo, hi) of the 256-thread block, x = key - xref
+//! and y = index - yref, reduced in a fixed order into out[] (thread 0).
 __device__ void unit_moments(const uint64_t* __restrict__ s, uint32_t lo, uint32_t hi,
-                             uint64_t xref, uint32_t yref, bool log, double* out)
+                             uint64_t xref, uint32_t yref, double* out)
 {
     __shared__ double red[4][NMOM];
     double m[NMOM] = { 0, 0, 0, 0, 0 };
     for (uint32_t i = lo + threadIdx.x; i < hi; i += 256)
     {
-        const uint64_t k = log ? log_image(s[i]) : s[i];
-        const double x = static_cast<double>(k - xref); // k >= xref: sorted, monotone image
+        const double x = static_cast<double>(s[i] - xref); // s[i] >= xref: sorted
         const double y = static_cast<double>(i - yref);
         m[0] += 1.0;
         m[1] += x;
@@ -156,43 +158,38 @@ __device__ void combine_moments(const double* __restrict__ part, uint32_t u0, ui
         for (int k = 0; k < NMOM; ++k) out[k] = red[0][k];
 }
 
-//! Root moments: unit blockIdx.x of the whole sample, relative to its first
-//! key (or its log image); part has root_units units per candidate.
-__global__ __launch_bounds__(256) void root_partial_kernel(const uint64_t* __restrict__ s,
-                                                           uint32_t m, uint32_t root_units,
-                                                           double* __restrict__ part)
-{
-    const bool log = blockIdx.y == 1;
-    const uint32_t lo = blockIdx.x * FIT_UNIT;
-    double mo[NMOM];
-    unit_moments(s, lo, min(m, lo + FIT_UNIT), log ? log_image(s[0]) : s[0], 0, log, mo);
-    if (threadIdx.x == 0)
-        for (int k = 0; k < NMOM; ++k)
-            part[(size_t(blockIdx.y) * root_units + blockIdx.x) * NMOM + k] = mo[k];
-}
-
-//! Root model: leaf = a0 * f + b0 maps the key's feature f to 256 leaves by
-//! its least-squares sample position.
-__global__ __launch_bounds__(256) void root_fit_kernel(const uint64_t* __restrict__ s,
-                                                       const double* __restrict__ part,
-                                                       uint32_t units, uint32_t m, uint32_t nb,
-                                                       rmi_model* models)
+//! Root model: leaf = a0 * f + b0 maps the key's feature f to 256 leaves
+//! by the secant of the sample's empirical CDF through its quantiles q and
+//! 1 - q (q = 1/512): samples between them spread over the leaves in
+//! proportion to their rank; the tails beyond land in the first and last
+//! leaves. A least-squares line over the whole sample instead (fitted to
+//! an S-shaped CDF, such as a lognormal's in the log image) clamped about
+//! 4% of the keys into each end leaf, where one linear leaf cannot follow
+//! the tail, and 3% of the keys of 2^28 lognormal(20, 2) keys went to
+//! overfull buckets. All keys at the two quantiles equal: one leaf.
+__global__ __launch_bounds__(64) void root_fit_kernel(const uint64_t* __restrict__ s,
+                                                      uint32_t m, uint32_t nb,
+                                                      rmi_model* models)
 {
     const uint32_t c = blockIdx.y;
-    double mo[NMOM];
-    combine_moments(part + size_t(c) * units * NMOM, 0, units, mo);
-    if (threadIdx.x == 0)
+    if (threadIdx.x != 0) return;
+    const uint32_t i1 = m / ROOT_QUANTILE, i2 = m - 1 - i1;
+    const uint64_t f1 = c == 1 ? log_image(s[i1]) : s[i1];
+    const uint64_t f2 = c == 1 ? log_image(s[i2]) : s[i2];
+    const double scale = double(LEAVES) / double(m);
+    rmi_model& M = models[c];
+    if (f2 > f1)
     {
-        double a, ci;
-        line_from_moments(mo, a, ci); // index ~ a * (f - f(s[0])) + ci
-        const double scale = double(LEAVES) / double(m);
-        const uint64_t fref = c == 1 ? log_image(s[0]) : s[0];
-        rmi_model& M = models[c];
-        M.a0 = a * scale;
-        M.b0 = (ci - a * static_cast<double>(fref)) * scale;
-        M.nb = double(nb);
-        M.root_log = c;
+        M.a0 = double(i2 - i1) * scale / double(f2 - f1);
+        M.b0 = double(i1) * scale - M.a0 * static_cast<double>(f1);
     }
+    else
+    {
+        M.a0 = 0.0;
+        M.b0 = double(LEAVES / 2);
+    }
+    M.nb = double(nb);
+    M.root_log = c;
 }
 
 //! lstart[l] = first sample index whose leaf is >= l (leaves are
@@ -229,8 +226,7 @@ __global__ __launch_bounds__(256) void leaf_units_kernel(const uint32_t* __restr
     if (l == LEAVES - 1) ustart[LEAVES] = before + u;
 }
 
-//! Leaf moments: one unit of one leaf, in the candidate's root feature (the
-//! key or its log image), relative to the leaf's first key's feature and
+//! Leaf moments: one unit of one leaf, relative to the leaf's first key and
 //! index. Launched with an upper bound of units (units_max per candidate);
 //! extra blocks return.
 __global__ __launch_bounds__(256) void leaf_partial_kernel(const uint64_t* __restrict__ s,
@@ -253,18 +249,16 @@ __global__ __launch_bounds__(256) void leaf_partial_kernel(const uint64_t* __res
     const uint32_t l = lo_l;
     const uint32_t first = lstart[l];
     const uint32_t lo = first + (b - ustart[l]) * FIT_UNIT;
-    const bool log = blockIdx.y == 1;
     double mo[NMOM];
-    unit_moments(s, lo, min(lstart[l + 1], lo + FIT_UNIT), log ? log_image(s[first]) : s[first],
-                 first, log, mo);
+    unit_moments(s, lo, min(lstart[l + 1], lo + FIT_UNIT), s[first], first, mo);
     if (threadIdx.x == 0)
         for (int k = 0; k < NMOM; ++k)
             part[(size_t(blockIdx.y) * units_max + b) * NMOM + k] = mo[k];
 }
 
-//! Leaf l: linear model of the empirical CDF in the root feature (slope
-//! clamped >= 0) and its bucket range [lo, hi] (proportional to its share
-//! of the sample; each range ends where the next begins).
+//! Leaf l: linear model of the empirical CDF (slope clamped >= 0) and its
+//! bucket range [lo, hi] (proportional to its share of the sample; each
+//! range ends where the next begins).
 __global__ __launch_bounds__(256) void leaf_fit_kernel(const uint64_t* __restrict__ s,
                                                        uint32_t m, uint32_t nb,
                                                        const uint32_t* __restrict__ lstart_all,
@@ -286,10 +280,9 @@ __global__ __launch_bounds__(256) void leaf_fit_kernel(const uint64_t* __restric
         if (hi - lo >= 2)
         {
             double ai, ci;
-            line_from_moments(mo, ai, ci); // index - lo ~ ai * (f - f(s[lo])) + ci
-            const uint64_t fref = c == 1 ? log_image(s[lo]) : s[lo];
+            line_from_moments(mo, ai, ci); // index - lo ~ ai * (x - s[lo]) + ci
             a = ai / double(m);
-            b = (double(lo) + ci - ai * static_cast<double>(fref)) / double(m);
+            b = (double(lo) + ci - ai * static_cast<double>(s[lo])) / double(m);
         }
         const uint32_t blo = static_cast<uint32_t>(uint64_t(nb) * lo / m);
         const uint32_t next =
@@ -725,9 +718,8 @@ void run(context& ctx, K* keys, uint32_t* vals, uint32_t n, int key_type, bool s
     const std::size_t o_model = L.take(NC * sizeof(rmi_model));
     const std::size_t o_lstart = L.take(NC * (LEAVES + 1) * 4);
     const std::size_t o_ustart = L.take(NC * (LEAVES + 1) * 4);
-    const uint32_t root_units = (m + FIT_UNIT - 1) / FIT_UNIT;
-    const uint32_t leaf_units_max = LEAVES + root_units; // a leaf adds at most one partial unit
-    const std::size_t o_rpart = L.take(std::size_t(NC) * root_units * NMOM * 8);
+    const uint32_t sample_units = (m + FIT_UNIT - 1) / FIT_UNIT;
+    const uint32_t leaf_units_max = LEAVES + sample_units; // a leaf adds at most one partial unit
     const std::size_t o_lpart = L.take(std::size_t(NC) * leaf_units_max * NMOM * 8);
     const std::size_t o_base = L.take(2 * RADIX * 4);
     const std::size_t o_sample = L.take(std::size_t(m) * 8);
@@ -746,7 +738,6 @@ void run(context& ctx, K* keys, uint32_t* vals, uint32_t n, int key_type, bool s
     rmi_model* models = reinterpret_cast<rmi_model*>(t + o_model);
     uint32_t* lstart = reinterpret_cast<uint32_t*>(t + o_lstart);
     uint32_t* ustart = reinterpret_cast<uint32_t*>(t + o_ustart);
-    double* rpart = reinterpret_cast<double*>(t + o_rpart);
     double* lpart = reinterpret_cast<double*>(t + o_lpart);
     uint32_t* base = reinterpret_cast<uint32_t*>(t + o_base);
     uint64_t* sample = reinterpret_cast<uint64_t*>(t + o_sample);
@@ -765,10 +756,7 @@ void run(context& ctx, K* keys, uint32_t* vals, uint32_t n, int key_type, bool s
         // Both candidate models at once: blockIdx.y selects the candidate.
         const uint32_t sgrid = std::max(1u, std::min<uint32_t>((m + 2047) / 2048,
                                                                uint32_t(ctx.facts().cu_count) * 4));
-        hipLaunchKernelGGL(root_partial_kernel, dim3(root_units, NC), dim3(256), 0, s, sample,
-                           m, root_units, rpart);
-        hipLaunchKernelGGL(root_fit_kernel, dim3(1, NC), dim3(256), 0, s, sample, rpart,
-                           root_units, m, nb, models);
+        hipLaunchKernelGGL(root_fit_kernel, dim3(1, NC), dim3(64), 0, s, sample, m, nb, models);
         hipLaunchKernelGGL(leaf_bounds_kernel, dim3(sgrid, NC), dim3(256), 0, s, sample, m,
                            models, lstart);
         hipLaunchKernelGGL(leaf_units_kernel, dim3(1, NC), dim3(256), 0, s, lstart, ustart);
