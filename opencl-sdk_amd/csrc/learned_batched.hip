@@ -52,6 +52,9 @@ namespace {
 using namespace dev;
 using namespace radix;
 
+//! The root's secant runs through the sample's quantiles 1/ROOT_QUANTILE
+//! and 1 - 1/ROOT_QUANTILE.
+constexpr uint32_t ROOT_QUANTILE = 512;
 //! Target keys per bucket: buckets near it fit the 4096-key LDS class.
 constexpr uint32_t BUCKET_TARGET = 3072;
 //! Smallest segment given a model (327 samples); smaller ones, and those
@@ -167,27 +170,10 @@ __global__ __launch_bounds__(256) void sample_kernel(const uint32_t* __restrict_
         out[i] = order_bits<uint32_t, KIND, DESC>(in[size_t(i) * SAMPLE_STRIDE + SAMPLE_STRIDE / 2]);
 }
 
-//! Sum of NMOM doubles over the 256-thread block in a fixed order; every
-//! thread gets it. red holds 4 * 5 doubles.
-__device__ void block_moments(double (&mo)[5], double (*red)[5])
-{
-    const uint32_t lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
-#pragma unroll
-    for (int k = 0; k < 5; ++k)
-    {
-#pragma unroll
-        for (int o = WAVE / 2; o > 0; o >>= 1) mo[k] += __shfl_xor(mo[k], o, WAVE);
-        if (lane == 0) red[wave][k] = mo[k];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < 5; ++k) mo[k] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
-    __syncthreads();
-}
-
 //! Block per learned segment: fits its RMI on the sorted sample and checks
 //! the fit. The root is linear in the 32-bit order image (for floats that
-//! image is already piecewise linear in log2 |x|). Each leaf is fitted by
+//! image is already piecewise linear in log2 |x|): the secant of the
+//! sample's CDF between two quantiles. Each leaf is fitted by
 //! one wave over its samples (least squares relative to the leaf's first
 //! sample). The check counts samples i whose bucket also holds sample i + K,
 //! K samples standing for cap keys, with a different key: those keys sit in
@@ -200,7 +186,6 @@ __global__ __launch_bounds__(256) void fit_kernel(const uint32_t* __restrict__ s
                                                   double fit_limit,
                                                   unsigned long long* __restrict__ fit_bad)
 {
-    __shared__ double red[4][5];
     __shared__ uint32_t lstart[LEAVES + 1];
     __shared__ rmi_model sm;
     __shared__ uint32_t s_cnt[4];
@@ -212,26 +197,23 @@ __global__ __launch_bounds__(256) void fit_kernel(const uint32_t* __restrict__ s
     const uint32_t* s = samples + so;
     const uint32_t tid = threadIdx.x;
 
-    // Root: index ~ a * (x - s[0]) + c over the whole sample.
-    double mo[5] = { 0, 0, 0, 0, 0 };
-    for (uint32_t i = tid; i < ns; i += 256)
-    {
-        const double x = static_cast<double>(s[i] - s[0]);
-        const double y = static_cast<double>(i);
-        mo[0] += 1.0;
-        mo[1] += x;
-        mo[2] += y;
-        mo[3] += x * x;
-        mo[4] += x * y;
-    }
-    block_moments(mo, red);
+    // Root: the secant of the sample's CDF through its quantiles q and 1 - q
+    // (q = 1/ROOT_QUANTILE), as in the single-array learned sort: the
+    // samples between them spread over the leaves by rank.
     if (tid == 0)
     {
-        double a, c;
-        line_from_moments(mo, a, c);
+        const uint32_t i1 = ns / ROOT_QUANTILE, i2 = ns - 1 - i1;
         const double scale = double(LEAVES) / double(ns);
-        sm.a0 = a * scale;
-        sm.b0 = (c - a * static_cast<double>(s[0])) * scale;
+        if (s[i2] > s[i1])
+        {
+            sm.a0 = double(i2 - i1) * scale / double(s[i2] - s[i1]);
+            sm.b0 = double(i1) * scale - sm.a0 * static_cast<double>(s[i1]);
+        }
+        else
+        {
+            sm.a0 = 0.0;
+            sm.b0 = double(LEAVES / 2);
+        }
         sm.nb = double(nb);
         sm.root_log = 0;
         sm.pad = 0;
