@@ -47,7 +47,7 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d)
     }
     uint32_t diff_lo = 0, diff_hi = 0;
 #pragma unroll
-    for (int b = 0; b < 8; ++b)
+    for (int b = 0; b < 7; ++b)
     {
         diff_lo |= static_cast<uint32_t>(bal[b]) ^ m[b];
         diff_hi |= static_cast<uint32_t>(bal[b] >> 32) ^ m[b];
@@ -55,7 +55,11 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d)
         // v_bitop3, rather than XORs re-associated into OR trees.
         asm("" : "+v"(diff_lo), "+v"(diff_hi));
     }
-    return ~((static_cast<uint64_t>(diff_hi) << 32) | diff_lo);
+    // The last fold also complements: ~(diff | (bal ^ m)) is one v_bitop3
+    // per half (a separate NOT cost two instructions a row: 2% of the pass).
+    const uint32_t lo = ~(diff_lo | (static_cast<uint32_t>(bal[7]) ^ m[7]));
+    const uint32_t hi = ~(diff_hi | (static_cast<uint32_t>(bal[7] >> 32) ^ m[7]));
+    return (static_cast<uint64_t>(hi) << 32) | lo;
 }
 
 //! Digit of a key for one pass: byte shift/8 of the key's order image.
@@ -180,6 +184,9 @@ __device__ __forceinline__ void onesweep_load(const K* __restrict__ in, uint32_t
 //! staging LDS of key-value tiles so larger tiles (longer contiguous digit
 //! runs in HBM) fit.
 //!
+//! db is the digit place's global start of digit tid (threads tid < 256;
+//! loaded by the caller before the tile, so its latency is hidden).
+//!
 //! A digit functor with STAGED = true is costly to evaluate: each key's
 //! digit is then computed once, kept (four to a register) for the reorder,
 //! and stored as a byte at its slot in dig.s_dig (TILE bytes of LDS) for
@@ -188,9 +195,9 @@ template <typename K, typename S, bool VALUES, int BLOCK, int ITEMS, bool FULL,
           bool SPLIT, typename DF>
 __device__ __forceinline__ void onesweep_rank_scatter(
     K (&k)[ITEMS], uint32_t (&v)[VALUES ? ITEMS : 1], K* __restrict__ keys_out,
-    uint32_t* __restrict__ vals_out, uint32_t n, DF dig,
-    const uint32_t* __restrict__ digit_base, S* status, uint32_t tile, uint32_t* s_hist,
-    K* s_keys, uint32_t* s_vals, uint32_t* s_gbase, uint32_t* s_tmp)
+    uint32_t* __restrict__ vals_out, uint32_t n, DF dig, uint32_t db, S* status,
+    uint32_t tile, uint32_t* s_hist, K* s_keys, uint32_t* s_vals, uint32_t* s_gbase,
+    uint32_t* s_tmp)
 {
     using SB = status_bits<S>;
     constexpr int W = BLOCK / WAVE;
@@ -247,7 +254,7 @@ __device__ __forceinline__ void onesweep_rank_scatter(
 #pragma unroll
         for (int w = 0; w < W; ++w) s_hist[w * RADIX + tid] += digit_start;
         s_total[tid] = total;
-        s_gbase[tid] = digit_base[tid] - digit_start; // the look-back adds the prefix
+        s_gbase[tid] = db - digit_start; // the look-back adds the prefix
     }
     __syncthreads();
 
@@ -381,6 +388,9 @@ __device__ __forceinline__ void onesweep_tile(
     S* status, uint32_t tile, uint32_t* s_hist, K* s_keys, uint32_t* s_vals,
     uint32_t* s_gbase, uint32_t* s_tmp)
 {
+    // Issued before the tile's loads: the digit scan, which uses it, then
+    // finds it long arrived (read there, its latency stalled the scan).
+    const uint32_t db = threadIdx.x < RADIX ? digit_base[threadIdx.x] : 0u;
     K k[ITEMS];
     uint32_t v[VALUES ? ITEMS : 1];
     onesweep_load<K, BLOCK, ITEMS, FULL>(keys_in, n, tile, k);
@@ -389,8 +399,8 @@ __device__ __forceinline__ void onesweep_tile(
     // polls (a wave waits for its vector memory operations in issue order).
     if constexpr (VALUES) onesweep_load<uint32_t, BLOCK, ITEMS, FULL>(vals_in, n, tile, v);
     onesweep_rank_scatter<K, S, VALUES, BLOCK, ITEMS, FULL, SPLIT>(
-        k, v, keys_out, vals_out, n, dig, digit_base, status, tile, s_hist, s_keys, s_vals,
-        s_gbase, s_tmp);
+        k, v, keys_out, vals_out, n, dig, db, status, tile, s_hist, s_keys, s_vals, s_gbase,
+        s_tmp);
 }
 
 // ---- whole segment in LDS ----------------------------------------------------
