@@ -608,7 +608,7 @@ clc_status clc_segmented_sort(clc_context ctx, void* keys, clc_type key_type,
     return guarded([&] {
         require(ctx != nullptr, "clc: null context");
         clc::detail::segmented_sort(get(ctx), keys, key_type, values, num_items,
-                                    offsets, num_segments, descending != 0);
+                                    offsets, num_segments, descending != 0, true);
     });
 }
 

@@ -6,14 +6,14 @@ namespace seg {
 template std::size_t sort_typed<uint32_t, dev::KEY_UNSIGNED>(context&, bool, void*, bool, void*,
                                                    std::size_t,
                                                       const uint64_t*, std::size_t, bool,
-                                                      const table&);
+                                                      const table&, bool);
 template std::size_t sort_typed<uint32_t, dev::KEY_SIGNED>(context&, bool, void*, bool, void*,
                                                    std::size_t,
                                                     const uint64_t*, std::size_t, bool,
-                                                    const table&);
+                                                    const table&, bool);
 template std::size_t sort_typed<uint32_t, dev::KEY_FLOAT>(context&, bool, void*, bool, void*,
                                                    std::size_t,
                                                    const uint64_t*, std::size_t, bool,
-                                                   const table&);
+                                                   const table&, bool);
 } // namespace seg
 } // namespace clc

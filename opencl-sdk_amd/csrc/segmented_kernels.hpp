@@ -606,7 +606,7 @@ template <typename F> void large_lds_opt_in(F kernel, std::size_t lds)
 template <typename K, int KIND, bool DESC, bool VALUES>
 std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
                       std::size_t num_items, const uint64_t* off, std::size_t nseg,
-                      const table& t)
+                      const table& t, bool overlap_large)
 {
     constexpr int P = sizeof(K);
     hipStream_t s = ctx.stream();
@@ -749,10 +749,9 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
     // class last (forked first it measured slower: 7.04 against 6.61 ms for
     // the learned sort of 2^28 lognormal keys, whose buckets fill the mid
     // class and three others; the classes then compete for the CUs from
-    // the start). The large path follows on s once they are done:
-    // overlapped with them as well, it measured slower on the lognormal
-    // batches (2.17 against 2.05 ms; its look-back chains stall behind
-    // other kernels' workgroups).
+    // the start). The large path runs on s: with overlap_large at once,
+    // beside the classes (the branches are joined after it), otherwise
+    // once they are done (see sort_typed for the measurements of both).
     hipEvent_t fork = ctx.aux_event(0);
     CLC_HIP(hipEventRecord(fork, s));
     int branch = 0;
@@ -762,6 +761,8 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
         hipStream_t prev;
         ~stream_swap() { c.swap_stream(prev); }
     };
+    hipEvent_t pending[context::AUX_EVENTS];
+    int npending = 0;
     auto run_branch = [&](auto&& body) {
         const hipStream_t as = ctx.aux_stream(branch % context::AUX_STREAMS);
         CLC_HIP(hipStreamWaitEvent(as, fork, 0));
@@ -771,7 +772,8 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
         }
         const hipEvent_t done = ctx.aux_event(1 + branch);
         CLC_HIP(hipEventRecord(done, as));
-        CLC_HIP(hipStreamWaitEvent(s, done, 0));
+        if (overlap_large) pending[npending++] = done;
+        else CLC_HIP(hipStreamWaitEvent(s, done, 0));
         ++branch;
     };
     static_assert(NCFG + 2 < context::AUX_EVENTS, "one event per branch and the fork");
@@ -787,6 +789,7 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
                 ctx, false, keys, vals, entries + std::size_t(NCFG) * nseg, slot(MID));
         });
     if (L_max > 0) large_path(s);
+    for (int i = 0; i < npending; ++i) CLC_HIP(hipStreamWaitEvent(s, pending[i], 0));
     CLC_HIP(hipGetLastError());
     return Lt.bytes;
 }
@@ -796,18 +799,19 @@ std::size_t sort_impl(context& ctx, bool prepare, K* keys, uint32_t* vals,
 template <typename K, int KIND>
 std::size_t sort_typed(context& ctx, bool prepare, void* keys, bool with_values,
                        void* values, std::size_t num_items, const uint64_t* offsets,
-                       std::size_t num_segments, bool descending, const table& t)
+                       std::size_t num_segments, bool descending, const table& t,
+                       bool overlap_large)
 {
     K* k = static_cast<K*>(keys);
     uint32_t* v = static_cast<uint32_t*>(values);
     const std::size_t n = num_items, ns = num_segments;
     if (descending)
     {
-        if (with_values) return sort_impl<K, KIND, true, true>(ctx, prepare, k, v, n, offsets, ns, t);
-        return sort_impl<K, KIND, true, false>(ctx, prepare, k, v, n, offsets, ns, t);
+        if (with_values) return sort_impl<K, KIND, true, true>(ctx, prepare, k, v, n, offsets, ns, t, overlap_large);
+        return sort_impl<K, KIND, true, false>(ctx, prepare, k, v, n, offsets, ns, t, overlap_large);
     }
-    if (with_values) return sort_impl<K, KIND, false, true>(ctx, prepare, k, v, n, offsets, ns, t);
-    return sort_impl<K, KIND, false, false>(ctx, prepare, k, v, n, offsets, ns, t);
+    if (with_values) return sort_impl<K, KIND, false, true>(ctx, prepare, k, v, n, offsets, ns, t, overlap_large);
+    return sort_impl<K, KIND, false, false>(ctx, prepare, k, v, n, offsets, ns, t, overlap_large);
 }
 
 } // namespace seg

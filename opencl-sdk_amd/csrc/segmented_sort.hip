@@ -38,7 +38,7 @@ table default_table(std::size_t kbytes, bool values, std::size_t lds_limit)
 namespace {
 
 using sort_fn = std::size_t (*)(context&, bool, void*, bool, void*, std::size_t,
-                                const uint64_t*, std::size_t, bool, const table&);
+                                const uint64_t*, std::size_t, bool, const table&, bool);
 
 sort_fn pick(int key_type)
 {
@@ -200,7 +200,7 @@ table run_tuner(context& ctx, int key_type, bool values)
                                    uint64_t(b * 131 + rep));
                 CLC_HIP(hipEventRecord(e0, s));
                 fn(ctx, false, keys.get(), values, vals.get(), n,
-                   offs.as<const uint64_t>(), nseg, false, t);
+                   offs.as<const uint64_t>(), nseg, false, t, false);
                 CLC_HIP(hipEventRecord(e1, s));
                 CLC_HIP(hipEventSynchronize(e1));
                 float ms = 0;
@@ -305,14 +305,14 @@ void check_args(int key_type, const void* keys, const uint64_t* offsets,
 
 void segmented_sort(context& ctx, void* keys, int key_type, void* values,
                     std::size_t num_items, const uint64_t* offsets,
-                    std::size_t num_segments, bool descending)
+                    std::size_t num_segments, bool descending, bool overlap_large)
 {
     check_args(key_type, keys, offsets, num_items, num_segments);
     if (num_segments == 0 || num_items < 2) return;
     ctx.activate();
     const seg::table t = seg::table_for(ctx, key_type, values != nullptr);
     seg::pick(key_type)(ctx, false, keys, values != nullptr, values, num_items, offsets,
-                        num_segments, descending, t);
+                        num_segments, descending, t, overlap_large);
 }
 
 void segmented_sort_tune(context& ctx, int key_type, bool with_values)
@@ -352,7 +352,7 @@ std::unique_ptr<seg::graph> segmented_sort_graph(context& ctx, void* keys, int k
     std::unique_ptr<seg::graph> g(new seg::graph);
     g->ctx = &ctx;
     const std::size_t bytes = fn(ctx, true, keys, values != nullptr, values, num_items,
-                                 offsets, num_segments, descending, t);
+                                 offsets, num_segments, descending, t, true);
     g->scratch.reset(new temp_buffer(ctx.resource(), bytes, ctx.stream()));
     CLC_HIP(hipStreamCreateWithFlags(&g->capture_stream, hipStreamNonBlocking));
     // Launch everything on a private stream under capture, drawing scratch
@@ -380,7 +380,7 @@ std::unique_ptr<seg::graph> segmented_sort_graph(context& ctx, void* keys, int k
     {
         if (num_segments > 0 && num_items >= 2)
             fn(ctx, false, keys, values != nullptr, values, num_items, offsets,
-               num_segments, descending, t);
+               num_segments, descending, t, true);
     } catch (...)
     {
         hipGraph_t broken = nullptr;

@@ -118,10 +118,18 @@ table default_table(std::size_t kbytes, bool values, std::size_t lds_limit);
 //! Enqueues the segmented sort for one key type on ctx's stream, or with
 //! prepare set only sizes its scratch and warms its launch state. Returns
 //! the scratch bytes. Instantiated in segmented_k32.hip / segmented_k64.hip.
+//! With overlap_large the onesweep passes of segments above every LDS
+//! capacity run while the LDS classes are still sorting (on the 65,536
+//! lognormal segments of the batched benchmark: 1.47 -> 1.38 ms); without
+//! it they follow the classes, which the learned sorts' bucket finish
+//! prefers (2^28 lognormal / Gaussian-mixture keys: 6.44 / 8.0 ms against
+//! 6.58 / 8.4 ms overlapped: their few overfull buckets' chains stall
+//! behind the classes' workgroups).
 template <typename K, int KIND>
 std::size_t sort_typed(context& ctx, bool prepare, void* keys, bool with_values,
                        void* values, std::size_t num_items, const uint64_t* offsets,
-                       std::size_t num_segments, bool descending, const table& t);
+                       std::size_t num_segments, bool descending, const table& t,
+                       bool overlap_large);
 
 //! A captured segmented sort: the whole launch sequence as one hipGraph,
 //! replayed on the context's stream. Its scratch is one block of the device
@@ -143,9 +151,11 @@ struct graph
 namespace detail {
 
 //! Stable sort of every segment [offsets[i], offsets[i+1]) of keys/values.
+//! overlap_large: see seg::sort_typed (the C entry point sets it; the
+//! learned sorts' bucket finish does not).
 void segmented_sort(context& ctx, void* keys, int key_type, void* values,
                     std::size_t num_items, const uint64_t* offsets,
-                    std::size_t num_segments, bool descending);
+                    std::size_t num_segments, bool descending, bool overlap_large = false);
 //! Runs the autotuner for a key type and stores its table in ctx (and on
 //! disk when ctx has a cache directory).
 void segmented_sort_tune(context& ctx, int key_type, bool with_values);
