@@ -1,6 +1,6 @@
 // Learned-bucket sort for gfx950 (no counterpart in the reference, whose
 // only sort is detail::radix_sort, Algorithm.cpp:453-527; the plain radix
-// sort of this library stays the fallback and the judge).
+// sort of this library stays the fallback and the bit-exact reference).
 //
 //   sample       every 100th key (its order image, as u64): a strided 1%
 //                sample, sorted by the plain radix sort

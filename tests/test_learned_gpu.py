@@ -2,7 +2,7 @@
 
 Every output must be bit-identical to the oracle's stable sort (keys and
 value permutation) and, at full size, to the library's plain radix sort
-(clc_sort), which stays the judge. Covers the three skewed u64 key
+(clc_sort), the bit-exact reference. Covers the three skewed u64 key
 distributions of the learned configuration (Zipf s=1.2 over [1, 2^40],
 lognormal(20, 2), a mixture of 8 Gaussians), uniform keys, signed and float
 keys in both directions, keys-only, degenerate inputs (one repeated key,
