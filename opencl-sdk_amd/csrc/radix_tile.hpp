@@ -289,6 +289,13 @@ __device__ __forceinline__ void onesweep_rank_scatter(
         if constexpr (STAGED) return dig.s_dig[i];
         else return dig(key);
     };
+    // 32-bit keys with 32-bit status words (fewer than 2^30 keys): every byte
+    // offset into the key and value arrays fits 32 bits, and a store from the
+    // uniform base with a 32-bit vector offset needs no 64-bit address add.
+    constexpr bool OFF32 = sizeof(K) == 4 && sizeof(S) == 4;
+    auto store_off = [](auto* base, uint32_t off, auto x) {
+        *reinterpret_cast<decltype(x)*>(reinterpret_cast<char*>(base) + off) = x;
+    };
     if constexpr (SPLIT && VALUES && sizeof(K) == 8)
     {
         // An 8-byte slot holds two words once its key is written out: the
@@ -321,6 +328,35 @@ __device__ __forceinline__ void onesweep_rank_scatter(
                 const uint2 dv = reinterpret_cast<const uint2*>(s_keys)[i];
                 vals_out[dv.x] = dv.y;
             }
+        }
+    }
+    else if constexpr (SPLIT && VALUES && OFF32)
+    {
+        // A thread writes the same slots (j * BLOCK + tid) in both loops: it
+        // keeps their byte offsets for the value pass (one register each).
+        uint32_t off[ITEMS];
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j)
+        {
+            const uint32_t i = j * BLOCK + tid;
+            if (FULL || i < count)
+            {
+                const K key = s_keys[i];
+                off[j] = (s_gbase[digit_at(i, key)] + i) * 4u;
+                store_off(keys_out, off[j], key);
+            }
+        }
+        __syncthreads(); // keys out of the buffer: reuse it for the values
+        uint32_t* s_v = reinterpret_cast<uint32_t*>(s_keys);
+#pragma unroll
+        for (int r = 0; r < ITEMS; ++r)
+            if (FULL || wave_base + r * WAVE + lane < n) s_v[rank[r]] = v[r];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j)
+        {
+            const uint32_t i = j * BLOCK + tid;
+            if (FULL || i < count) store_off(vals_out, off[j], s_v[i]);
         }
     }
     else if constexpr (SPLIT && VALUES)
@@ -371,8 +407,16 @@ __device__ __forceinline__ void onesweep_rank_scatter(
             {
                 const K key = s_keys[i];
                 const uint32_t dst = s_gbase[digit_at(i, key)] + i;
-                keys_out[dst] = key;
-                if constexpr (VALUES) vals_out[dst] = s_vals[i];
+                if constexpr (OFF32)
+                {
+                    store_off(keys_out, dst * 4u, key);
+                    if constexpr (VALUES) store_off(vals_out, dst * 4u, s_vals[i]);
+                }
+                else
+                {
+                    keys_out[dst] = key;
+                    if constexpr (VALUES) vals_out[dst] = s_vals[i];
+                }
             }
         }
     }
