@@ -1,4 +1,4 @@
-"""Sizes at and above 2^30 keys on the GPU, through the C ABI.
+"""Sizes around and above 2^30 keys on the GPU, through the C ABI.
 
 clc_sort takes up to 2^31-1 keys per call (the reference's check_size,
 lib/src/Compute/Detail.hpp:18-23). From 2^30 keys on, a digit count no
@@ -54,6 +54,33 @@ def test_sort_u32_pairs_2p30_plus_ragged_tail(ctx):
     check_stable_sort_properties(orig, keys, vals)
 
 
+@pytest.mark.parametrize("with_values", [True, False])
+def test_sort_u32_2p30_minus_one(ctx, with_values):
+    """The largest size on 32-bit status words: the write-out's 32-bit byte
+    offsets from the output bases reach 2^32 - 4 here."""
+    import torch
+    n = (1 << 30) - 1
+    say("sort u32, 32-bit status words, values" if with_values else "sort u32 keys only", n)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(34)
+    keys = torch.randint(-(1 << 31), (1 << 31) - 1, (n,), dtype=torch.int32,
+                         device="cuda", generator=g)
+    orig = keys.clone()
+    if with_values:
+        vals = torch.arange(n, dtype=torch.int32, device="cuda")
+        ctx.sort(keys, vals, key_type=clc_hip.U32)
+        torch.cuda.synchronize()
+        check_stable_sort_properties(orig, keys, vals)
+    else:
+        ctx.sort(keys, key_type=clc_hip.U32)
+        torch.cuda.synchronize()
+        # Keys only: ascending, and the same multiset as the input.
+        ku = keys.to(torch.int64) & 0xFFFFFFFF
+        assert bool((ku[1:] >= ku[:-1]).all())
+        ou = orig.to(torch.int64) & 0xFFFFFFFF
+        assert bool(torch.equal(torch.sort(ou).values, ku))
+
+
 def test_sort_u64_pairs_2p30_plus_one(ctx):
     import torch
     n = (1 << 30) + 1
@@ -102,6 +129,32 @@ def test_segment_of_2p30_keys_is_left_untouched(ctx):
     assert bool(torch.equal(keys[b:e], orig[b:e]))
     assert bool(torch.equal(vals[b:e], torch.arange(b, e, dtype=torch.int32, device="cuda")))
     # The small segments either side are sorted as the oracle sorts them.
+    for lo, hi in ((0, b), (e, n)):
+        k0 = host(orig[lo:hi].contiguous(), np.uint32)
+        ek, ev = oracle.radix_sort(k0, np.arange(lo, hi, dtype=np.uint32))
+        np.testing.assert_array_equal(host(keys[lo:hi].contiguous(), np.uint32), ek)
+        np.testing.assert_array_equal(host(vals[lo:hi].contiguous(), np.uint32), ev)
+
+
+def test_segment_of_2p30_minus_one_keys_is_sorted(ctx):
+    """The largest segment the segmented sort takes: its large path's 32-bit
+    byte offsets from the segment's base reach 2^32 - 4."""
+    import torch
+    big = (1 << 30) - 1
+    sizes = [1000, big, 1999]
+    n = sum(sizes)
+    say("segmented sort with a segment of 2^30-1 keys", n)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(35)
+    keys = torch.randint(-(1 << 31), (1 << 31) - 1, (n,), dtype=torch.int32,
+                         device="cuda", generator=g)
+    vals = torch.arange(n, dtype=torch.int32, device="cuda")
+    orig = keys.clone()
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    ctx.segmented_sort(keys, vals, dev(off), key_type=clc_hip.U32)
+    torch.cuda.synchronize()
+    b, e = int(off[1]), int(off[2])
+    check_stable_sort_properties(orig[b:e], keys[b:e], vals[b:e] - b)
     for lo, hi in ((0, b), (e, n)):
         k0 = host(orig[lo:hi].contiguous(), np.uint32)
         ek, ev = oracle.radix_sort(k0, np.arange(lo, hi, dtype=np.uint32))
