@@ -60,17 +60,19 @@ inline int os_default(std::size_t kbytes, bool values)
     return kbytes == 4 && os_feasible(6, kbytes, values) ? 6 : 0;
 }
 
-//! One digit place. The keys and values live in two buffers (a = the
-//! caller's, b = scratch); which one the pass reads is decided on the device:
-//! passes whose digit is the same for every key (trivial[p] != 0, set by the
-//! digit scan) are skipped, and each executed pass reads the buffer the
-//! previous executed pass wrote.
+//! One digit place. The keys and values live in two or three buffers (a =
+//! the caller's, b and c = scratch, c optional: null); which one the pass
+//! reads is decided on the device (pass_route): passes whose digit is the
+//! same for every key (trivial[p] != 0, set by the digit scan) are skipped,
+//! and each executed pass reads the buffer the previous executed pass wrote.
 struct onesweep_args
 {
     void* keys_a;
     void* keys_b;
+    void* keys_c;
     uint32_t* vals_a;
     uint32_t* vals_b;
+    uint32_t* vals_c;
     uint32_t n;
     uint32_t pass;
     const uint32_t* trivial; // one flag per digit place
@@ -87,6 +89,30 @@ template <typename K, int KIND>
 void launch_onesweep(int cfg, bool descending, bool values, bool wide_status,
                      const onesweep_args& a, hipStream_t s);
 
+//! The buffers an executed pass reads and writes (0 = a, 1 = b, 2 = c), from
+//! the number of executed passes before it (j) and in all (e). Two buffers
+//! alternate a -> b -> a ..., which leaves the result of an odd count in b
+//! (the caller then copies it back). With c, an odd count of at least three
+//! goes a -> b, then b -> c and c -> b in turn, and last c -> a, so the
+//! result lands in a: no copy after the passes.
+struct pass_route
+{
+    uint32_t src, dst;
+    __device__ __forceinline__ pass_route(uint32_t j, uint32_t e, bool three)
+    {
+        if (!three || (e & 1u) == 0 || e == 1) { src = j & 1u; dst = src ^ 1u; }
+        else if (j == 0) { src = 0; dst = 1; }
+        else if (j + 1 == e) { src = 2; dst = 0; }
+        else { src = (j & 1u) ? 1u : 2u; dst = 3u - src; }
+    }
+    //! Buffer i of (a, b, c), by selects (no indexed private array).
+    template <typename T>
+    __device__ __forceinline__ static T* pick(uint32_t i, T* a, T* b, T* c)
+    {
+        return i == 0 ? a : (i == 1 ? b : c);
+    }
+};
+
 //! One tile per workgroup, taken in launch order from an agent-scope
 //! counter, so a tile only waits on tiles that are already running (no
 //! co-residency is assumed). Not persistent on purpose: on gfx9 a wave's
@@ -96,8 +122,8 @@ void launch_onesweep(int cfg, bool descending, bool values, bool wide_status,
 template <typename K, typename S, int KIND, bool DESC, bool VALUES, int BLOCK,
           int ITEMS, bool SPLIT, int WPE>
 __global__ __launch_bounds__(BLOCK, WPE) void onesweep_kernel(
-    K* keys_a, K* keys_b, uint32_t* vals_a, uint32_t* vals_b, uint32_t n,
-    uint32_t pass, const uint32_t* __restrict__ trivial,
+    K* keys_a, K* keys_b, K* keys_c, uint32_t* vals_a, uint32_t* vals_b, uint32_t* vals_c,
+    uint32_t n, uint32_t pass, const uint32_t* __restrict__ trivial,
     const uint32_t* __restrict__ digit_base, S* status, S* status_next,
     uint32_t* tile_counter)
 {
@@ -132,12 +158,13 @@ __global__ __launch_bounds__(BLOCK, WPE) void onesweep_kernel(
             status_next[static_cast<size_t>(blockIdx.x) * RADIX + tid] = 0;
         return;
     }
-    // Executed passes before this one, mod 2, choose the buffers.
-    const uint32_t parity = __popc(~skip & ((1u << pass) - 1u)) & 1u;
-    const K* keys_in = parity ? keys_b : keys_a;
-    K* keys_out = parity ? keys_a : keys_b;
-    const uint32_t* vals_in = parity ? vals_b : vals_a;
-    uint32_t* vals_out = parity ? vals_a : vals_b;
+    // Executed passes before this one and in all choose the buffers.
+    const uint32_t ran = ~skip & ((1u << sizeof(K)) - 1u);
+    const pass_route route(__popc(ran & ((1u << pass) - 1u)), __popc(ran), keys_c != nullptr);
+    const K* keys_in = pass_route::pick(route.src, keys_a, keys_b, keys_c);
+    K* keys_out = pass_route::pick(route.dst, keys_a, keys_b, keys_c);
+    const uint32_t* vals_in = pass_route::pick(route.src, vals_a, vals_b, vals_c);
+    uint32_t* vals_out = pass_route::pick(route.dst, vals_a, vals_b, vals_c);
 
     for (uint32_t i = tid; i < W * RADIX; i += BLOCK) s_hist[i] = 0;
     __syncthreads();
@@ -284,8 +311,8 @@ template <typename T, int BLOCK, int ITEMS, bool BUF> struct pipe_loader
 
 template <typename K, typename S, int KIND, bool DESC, bool VALUES, int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK, BLOCK / 256) void onesweep_pipe_kernel(
-    K* keys_a, K* keys_b, uint32_t* vals_a, uint32_t* vals_b, uint32_t n,
-    uint32_t pass, const uint32_t* __restrict__ trivial,
+    K* keys_a, K* keys_b, K* keys_c, uint32_t* vals_a, uint32_t* vals_b, uint32_t* vals_c,
+    uint32_t n, uint32_t pass, const uint32_t* __restrict__ trivial,
     const uint32_t* __restrict__ digit_base, S* status, S* status_next,
     uint32_t* tile_counter)
 {
@@ -329,11 +356,13 @@ __global__ __launch_bounds__(BLOCK, BLOCK / 256) void onesweep_pipe_kernel(
                 status_next[i] = 0;
         return;
     }
-    const uint32_t parity = __popc(~skip & ((1u << pass) - 1u)) & 1u;
-    const K* keys_in = parity ? keys_b : keys_a;
-    K* keys_out = parity ? keys_a : keys_b;
-    const uint32_t* vals_in = parity ? vals_b : vals_a;
-    uint32_t* vals_out = parity ? vals_a : vals_b;
+    // Executed passes before this one and in all choose the buffers.
+    const uint32_t ran = ~skip & ((1u << sizeof(K)) - 1u);
+    const pass_route route(__popc(ran & ((1u << pass) - 1u)), __popc(ran), keys_c != nullptr);
+    const K* keys_in = pass_route::pick(route.src, keys_a, keys_b, keys_c);
+    K* keys_out = pass_route::pick(route.dst, keys_a, keys_b, keys_c);
+    const uint32_t* vals_in = pass_route::pick(route.src, vals_a, vals_b, vals_c);
+    uint32_t* vals_out = pass_route::pick(route.dst, vals_a, vals_b, vals_c);
     const key_digit<K, KIND, DESC> dig{ 8u * pass };
     const pipe_loader<K, BLOCK, ITEMS, OFF32> load_keys(keys_in, n);
     const pipe_loader<uint32_t, BLOCK, ITEMS, OFF32> load_vals(vals_in, n);
@@ -501,8 +530,9 @@ void launch(const onesweep_args& a, hipStream_t s)
         }
         const uint32_t groups = std::max(1u, std::min(tiles, a.max_groups));
         hipLaunchKernelGGL(kernel, dim3(groups), dim3(BLOCK), lds, s,
-                           static_cast<K*>(a.keys_a), static_cast<K*>(a.keys_b), a.vals_a,
-                           a.vals_b, a.n, a.pass, a.trivial, a.digit_base,
+                           static_cast<K*>(a.keys_a), static_cast<K*>(a.keys_b),
+                           static_cast<K*>(a.keys_c), a.vals_a, a.vals_b, a.vals_c, a.n,
+                           a.pass, a.trivial, a.digit_base,
                            static_cast<S*>(a.status), static_cast<S*>(a.status_next),
                            a.tile_counter);
         return;
@@ -518,8 +548,9 @@ void launch(const onesweep_args& a, hipStream_t s)
         opted = true;
     }
     hipLaunchKernelGGL(kernel, dim3(tiles), dim3(BLOCK), lds, s,
-                       static_cast<K*>(a.keys_a), static_cast<K*>(a.keys_b), a.vals_a,
-                       a.vals_b, a.n, a.pass, a.trivial, a.digit_base,
+                       static_cast<K*>(a.keys_a), static_cast<K*>(a.keys_b),
+                       static_cast<K*>(a.keys_c), a.vals_a, a.vals_b, a.vals_c, a.n, a.pass,
+                       a.trivial, a.digit_base,
                        static_cast<S*>(a.status), static_cast<S*>(a.status_next),
                        a.tile_counter);
 }

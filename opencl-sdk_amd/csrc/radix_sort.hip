@@ -23,9 +23,12 @@
 // 64-key round, rounds in order), across waves (wave-ordered prefixes) and
 // across tiles (look-back), which keeps the sort stable. A digit place where
 // every key has the same digit is skipped on the device (its pass would be
-// the identity); each executed pass reads what the previous one wrote, and
-// if an odd number ran, a final copy returns the result to the caller's
-// buffers (in place, as Algorithm.cpp:526). No host round trip is needed.
+// the identity); each executed pass reads what the previous one wrote. The
+// result ends in the caller's buffers (in place, as Algorithm.cpp:526): an
+// odd number of executed passes routes through a second scratch buffer so
+// the last pass writes there (radix::pass_route), and only a single executed
+// pass, or a pool without room for that buffer, needs a final copy. No host
+// round trip is needed.
 #include "context.hpp"
 #include "device_common.hpp"
 #include "ops.hpp"
@@ -159,17 +162,19 @@ __global__ __launch_bounds__(256) void digit_scan_kernel(
     if (c == n) trivial[blockIdx.x] = 1;
 }
 
-//! After the passes: if an odd number of them ran, the result sits in the
-//! scratch buffers b; copy it back to the caller's a.
+//! After the passes: where the routing of the passes left the result in the
+//! scratch buffers b (an odd number of executed passes, and either just one
+//! or no second scratch c; see radix::pass_route), copy it back to the
+//! caller's a.
 template <typename K>
 __global__ __launch_bounds__(256) void finish_copy_kernel(
     K* __restrict__ keys_a, const K* __restrict__ keys_b, uint32_t* __restrict__ vals_a,
     const uint32_t* __restrict__ vals_b, uint32_t n, const uint32_t* __restrict__ trivial,
-    uint32_t passes)
+    uint32_t passes, uint32_t three)
 {
-    uint32_t odd = 0;
-    for (uint32_t p = 0; p < passes; ++p) odd ^= trivial[p] == 0 ? 1u : 0u;
-    if (!odd) return;
+    uint32_t ran = 0;
+    for (uint32_t p = 0; p < passes; ++p) ran += trivial[p] == 0 ? 1u : 0u;
+    if ((ran & 1u) == 0 || (three && ran > 1)) return;
     const uint32_t stride = gridDim.x * 256;
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
     {
@@ -234,15 +239,34 @@ void sort_impl(context& ctx, K* keys, uint32_t* vals, uint32_t n, bool desc, int
     const std::size_t o_base = L.take(P * RADIX * 4);
     const std::size_t o_kalt = L.take(std::size_t(n) * sizeof(K));
     const std::size_t o_valt = values ? L.take(std::size_t(n) * 4) : 0;
-    temp_buffer tmp(ctx.resource(), L.bytes, ctx.stream());
-    char* t = tmp.as<char>();
+    // A second scratch pair (c) lets an odd number of executed passes end in
+    // the caller's buffers (radix::pass_route); where the pool cannot give
+    // the memory, the passes alternate between two buffers and an odd count
+    // is copied back at the end (about two thirds of a pass).
+    layout L3 = L;
+    const std::size_t o_kalt2 = L3.take(std::size_t(n) * sizeof(K));
+    const std::size_t o_valt2 = values ? L3.take(std::size_t(n) * 4) : 0;
+    std::unique_ptr<temp_buffer> tmp;
+    bool three = true;
+    try
+    {
+        tmp = std::make_unique<temp_buffer>(ctx.resource(), L3.bytes, ctx.stream());
+    }
+    catch (const out_of_memory&)
+    {
+        three = false;
+        tmp = std::make_unique<temp_buffer>(ctx.resource(), L.bytes, ctx.stream());
+    }
+    char* t = tmp->as<char>();
     uint32_t* hist = reinterpret_cast<uint32_t*>(t + o_hist);
     uint32_t* ctr = reinterpret_cast<uint32_t*>(t + o_ctr);
     uint32_t* trivial = reinterpret_cast<uint32_t*>(t + o_triv);
     void* st[2] = { t + o_st0, t + o_st1 };
     uint32_t* base = reinterpret_cast<uint32_t*>(t + o_base);
-    K* kb[2] = { keys, reinterpret_cast<K*>(t + o_kalt) };
-    uint32_t* vb[2] = { vals, values ? reinterpret_cast<uint32_t*>(t + o_valt) : nullptr };
+    K* kb[3] = { keys, reinterpret_cast<K*>(t + o_kalt),
+                 three ? reinterpret_cast<K*>(t + o_kalt2) : nullptr };
+    uint32_t* vb[3] = { vals, values ? reinterpret_cast<uint32_t*>(t + o_valt) : nullptr,
+                        values && three ? reinterpret_cast<uint32_t*>(t + o_valt2) : nullptr };
 
     detail::zero_fill(s, t, zero_bytes);
     const uint32_t hist_grid =
@@ -263,7 +287,7 @@ void sort_impl(context& ctx, K* keys, uint32_t* vals, uint32_t n, bool desc, int
     for (int p = 0; p < P; ++p)
     {
         timed_launch timer(ctx, values ? "radix_onesweep_pairs" : "radix_onesweep_keys");
-        onesweep_args a{ kb[0], kb[1], vb[0], vb[1], n, uint32_t(p), trivial,
+        onesweep_args a{ kb[0], kb[1], kb[2], vb[0], vb[1], vb[2], n, uint32_t(p), trivial,
                          base + p * RADIX, st[p % 2],
                          p + 1 < P ? st[(p + 1) % 2] : nullptr, ctr + p,
                          uint32_t(ctx.facts().cu_count) };
@@ -272,7 +296,7 @@ void sort_impl(context& ctx, K* keys, uint32_t* vals, uint32_t n, bool desc, int
     const uint32_t copy_grid =
         std::max(1u, std::min<uint32_t>((n + 255) / 256, uint32_t(ctx.facts().cu_count) * 4));
     hipLaunchKernelGGL((finish_copy_kernel<K>), dim3(copy_grid), dim3(256), 0, s, kb[0],
-                       kb[1], vb[0], vb[1], n, trivial, uint32_t(P));
+                       kb[1], vb[0], vb[1], n, trivial, uint32_t(P), three ? 1u : 0u);
     CLC_HIP(hipGetLastError());
 }
 

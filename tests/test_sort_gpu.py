@@ -178,19 +178,43 @@ def test_skewed_inputs(ctx, case):
                                         (np.int32, 20), (np.uint64, 40), (np.uint64, 48),
                                         (np.uint64, 56), (np.int64, 12)])
 @pytest.mark.parametrize("descending", [False, True])
-def test_trivial_digit_places_skipped(ctx, dtype, bits, descending):
+@pytest.mark.parametrize("with_values", [True, False])
+def test_trivial_digit_places_skipped(ctx, dtype, bits, descending, with_values):
     """Keys confined to their low `bits` leave the upper digit places
     constant, so those passes are skipped on the device; odd and even counts
-    of executed passes must both land the result in the caller's buffers."""
+    of executed passes must both land the result in the caller's buffers
+    (one executed pass through a final copy, three or more through the
+    second scratch buffer)."""
     n = 150001
-    say("sort trivial places", np.dtype(dtype).name, bits, descending)
+    say("sort trivial places", np.dtype(dtype).name, bits, descending, with_values)
     rng = np.random.default_rng(bits)
     keys = rng.integers(0, 1 << bits, size=n, dtype=np.uint64).astype(dtype)
     vals = np.arange(n, dtype=np.uint32)
-    k, v = gpu_sort(ctx, keys, vals, descending)
+    k, v = gpu_sort(ctx, keys, vals if with_values else None, descending)
     ek, ev = oracle.radix_sort(keys, vals, descending=descending, threads=8)
     assert_same(k, ek)
-    np.testing.assert_array_equal(v, ev)
+    if with_values:
+        np.testing.assert_array_equal(v, ev)
+
+
+@pytest.mark.parametrize("shape", range(11))
+@pytest.mark.parametrize("dtype,bits", [(np.uint32, 24), (np.uint64, 40)])
+@pytest.mark.parametrize("with_values", [True, False])
+def test_every_tile_shape_odd_passes(ctx, monkeypatch, shape, dtype, bits, with_values):
+    """Three and five executed digit places (an odd count, routed through the
+    second scratch buffer) on every tile shape, the pipelined ones included."""
+    monkeypatch.setenv("CLC_ONESWEEP_CFG", str(shape))
+    for n in (1000, 300007):
+        say("sort shape, odd passes", shape, np.dtype(dtype).name, bits, with_values, n)
+        rng = np.random.default_rng(shape * 7 + n)
+        keys = rng.integers(0, 1 << bits, size=n, dtype=np.uint64).astype(dtype)
+        keys[:5] = (1 << bits) - 1  # the top executed place varies
+        vals = np.arange(n, dtype=np.uint32)
+        k, v = gpu_sort(ctx, keys, vals if with_values else None)
+        ek, ev = oracle.radix_sort(keys, vals, threads=8)
+        assert_same(k, ek)
+        if with_values:
+            np.testing.assert_array_equal(v, ev)
 
 
 def test_unaligned_pointer(ctx):
