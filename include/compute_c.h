@@ -392,7 +392,8 @@ typedef struct clc_learned_stats
 
 /* Same result as clc_sort (stable, same key order, values permuted
  * alongside), computed through a learned CDF model: a two-level RMI fitted
- * to a strided 1% sample (sorted by the plain radix sort) maps keys
+ * to a strided sample (every 100th key up to 65,536 samples, every 400th
+ * beyond; sorted by the plain radix sort) maps keys
  * monotonically to up to 65536 buckets, two stable passes keyed by the
  * bucket id group the keys, and each bucket is finished in LDS; buckets too
  * large for LDS, and inputs whose sample fits badly, go to the plain radix

@@ -11,7 +11,8 @@
 //                count (1%), its bucket count (one per ~3072 keys) and its
 //                slots in the bucket layout; exclusive scans of these and
 //                the onesweep tile layout of the learned segments
-//   sample       every 100th key's order image of each learned segment
+//   sample       a strided sample of each learned segment's order images
+//                (sample_count: every 100th key up to 65,536, every 400th beyond)
 //   sort         the segmented sort over the samples (one segment each)
 //   fit          one launch, one workgroup per segment: the segment's RMI
 //                (root linear model over the order image to 256 leaves,
@@ -106,7 +107,7 @@ __global__ __launch_bounds__(1024) void prepare_kernel(
             if (si.learned)
             {
                 si.nb = min(MAX_BUCKETS, (si.m + BUCKET_TARGET - 1) / BUCKET_TARGET);
-                x[0] = si.m / SAMPLE_STRIDE;
+                x[0] = sample_count(si.m);
                 x[1] = si.nb;
                 x[2] = (si.m + BTILE - 1) / BTILE;
             }
@@ -153,7 +154,7 @@ __global__ __launch_bounds__(1024) void prepare_kernel(
     }
 }
 
-//! Block per segment: the strided 1% sample of a learned segment (order
+//! Block per segment: the strided sample of a learned segment (order
 //! images; they sort ascending as unsigned 32-bit keys).
 template <int KIND, bool DESC>
 __global__ __launch_bounds__(256) void sample_kernel(const uint32_t* __restrict__ keys,
@@ -163,11 +164,11 @@ __global__ __launch_bounds__(256) void sample_kernel(const uint32_t* __restrict_
 {
     const seg_info si = info[blockIdx.x];
     if (!si.learned) return;
-    const uint32_t ns = si.m / SAMPLE_STRIDE;
+    const uint32_t ns = sample_count(si.m), every = si.m / ns; // si.m >= MIN_LEARNED
     uint32_t* out = samples + sample_off[blockIdx.x];
     const uint32_t* in = keys + si.begin;
     for (uint32_t i = threadIdx.x; i < ns; i += 256)
-        out[i] = order_bits<uint32_t, KIND, DESC>(in[size_t(i) * SAMPLE_STRIDE + SAMPLE_STRIDE / 2]);
+        out[i] = order_bits<uint32_t, KIND, DESC>(in[size_t(i) * every + every / 2]);
 }
 
 //! Block per learned segment: fits its RMI on the sorted sample and checks
@@ -576,7 +577,7 @@ void run(context& ctx, uint32_t* keys, uint32_t* vals, std::size_t n, const uint
     const uint32_t cap = static_cast<uint32_t>(std::min<std::size_t>(lds_cap(table), 1u << 30));
 
     // Upper bounds of the device-computed totals.
-    const std::size_t samples_max = n / SAMPLE_STRIDE + 1;
+    const std::size_t samples_max = n / SAMPLE_STRIDE_MIN + 1;
     const std::size_t slots_max = std::size_t(nseg) + n / BUCKET_TARGET + 1;
     const std::size_t tiles_max = n / BTILE + nseg + 1;
     if (slots_max >= (std::size_t(1) << 32) || tiles_max >= (std::size_t(1) << 31))

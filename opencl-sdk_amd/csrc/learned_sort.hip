@@ -2,8 +2,9 @@
 // only sort is detail::radix_sort, Algorithm.cpp:453-527; the plain radix
 // sort of this library stays the fallback and the bit-exact reference).
 //
-//   sample       every 100th key (its order image, as u64): a strided 1%
-//                sample, sorted by the plain radix sort
+//   sample       a strided sample of the keys' order images (as u64; every
+//                100th key up to 65,536, every 400th beyond: sample_count),
+//                sorted by the plain radix sort
 //   fit          a two-level RMI: one root linear model mapping the key to
 //                one of 256 leaves (the secant of the sample's CDF between
 //                two quantiles), and per leaf a linear model of the
@@ -73,6 +74,7 @@ constexpr uint32_t LTILE = LBLOCK * LITEMS;
 
 template <typename K, int KIND, bool DESC>
 __global__ __launch_bounds__(256) void sample_kernel(const K* __restrict__ keys, uint32_t m,
+                                                     uint32_t every,
                                                      uint64_t* __restrict__ out)
 {
     // Four strided reads per thread in flight (each its own cache line).
@@ -83,7 +85,7 @@ __global__ __launch_bounds__(256) void sample_kernel(const K* __restrict__ keys,
         K k[U];
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u)
-            k[u] = keys[size_t(min(i + u * stride, m - 1)) * SAMPLE_STRIDE + SAMPLE_STRIDE / 2];
+            k[u] = keys[size_t(min(i + u * stride, m - 1)) * every + every / 2];
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u)
             if (i + u * stride < m)
@@ -693,7 +695,7 @@ void run(context& ctx, K* keys, uint32_t* vals, uint32_t n, int key_type, bool s
          stats* out)
 {
     hipStream_t s = ctx.stream();
-    const uint32_t m = n / SAMPLE_STRIDE;
+    const uint32_t m = sample_count(n);
     // Buckets: about 4096 keys each, 512 to 65536 (two 8-bit passes), or at
     // most 4096 for the single scatter.
     const uint32_t nb_max = single ? SS_MAX_BUCKETS : 65536u;
@@ -751,7 +753,7 @@ void run(context& ctx, K* keys, uint32_t* vals, uint32_t n, int key_type, bool s
     {
         timed_launch timer(ctx, "learned_sample_fit");
         hipLaunchKernelGGL((sample_kernel<K, KIND, DESC>), dim3(grid), dim3(256), 0, s, keys,
-                           m, sample);
+                           m, n / m, sample);
         detail::radix_sort(ctx, sample, TY_U64, nullptr, m, false);
         // Both candidate models at once: blockIdx.y selects the candidate.
         const uint32_t sgrid = std::max(1u, std::min<uint32_t>((m + 2047) / 2048,
