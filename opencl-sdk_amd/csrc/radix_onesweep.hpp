@@ -227,17 +227,10 @@ __device__ __forceinline__ void pipe_rank(const K (&k)[ITEMS], uint32_t (&rank)[
     for (int r = 0; r < ITEMS; ++r)
     {
         const uint32_t d = dig(k[r]);
-        uint64_t peers = match_digit(d);
-        bool valid = true;
-        if constexpr (!FULL)
-        {
-            valid = wave_base + r * WAVE + lane < n;
-            peers &= __ballot(valid);
-        }
-        const uint32_t pre = count_below(peers);
-        const uint32_t base = my_hist[d];
-        if (valid && pre == 0) my_hist[d] = base + __popcll(peers);
-        rank[r] = base + pre;
+        // Rank: one LDS atomic add per key (see onesweep_rank_scatter).
+        if (FULL || wave_base + r * WAVE + lane < n)
+            rank[r] = __hip_atomic_fetch_add(&my_hist[d], 1u, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     __syncthreads();
     uint32_t total = 0;

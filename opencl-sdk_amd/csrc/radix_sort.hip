@@ -12,8 +12,8 @@
 //   digit_histogram  one read of the keys counts all sizeof(K) digit places
 //                    at once (LDS counters, one global add per bin per block)
 //   digit_scan       exclusive scan of each place's 256 counts
-//   onesweep (x P)   per tile: wave64 ballot match-and-rank, per-wave digit
-//                    counters in LDS, decoupled look-back across tiles per
+//   onesweep (x P)   per tile: ranks from per-wave digit counters in LDS
+//                    (one atomic add per key), decoupled look-back across tiles per
 //                    digit, tile reordered in LDS by digit and written to HBM
 //                    in contiguous digit runs.
 //

@@ -152,8 +152,8 @@ __global__ void tune_fill_kernel(uint32_t* words, std::size_t n, uint64_t seed)
 
 //! Times every feasible class for each size bucket on synthetic segments of
 //! that bucket and keeps the fastest; for an LDS class then the widest
-//! digit of its passes, from 8 bits down to 4 (uniform 32-bit keys: a
-//! narrower digit means more passes and fewer ballots per pass).
+//! digit of its passes, from 8 bits down to 4 (a narrower digit means more
+//! passes over fewer distinct counters).
 table run_tuner(context& ctx, int key_type, bool values)
 {
     const std::size_t kb = key_bytes(key_type);
