@@ -223,14 +223,13 @@ __device__ __forceinline__ void pipe_rank(const K (&k)[ITEMS], uint32_t (&rank)[
     const uint32_t tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
     const uint32_t wave_base = tile * TILE + wave * (WAVE * ITEMS);
     uint32_t* my_hist = s_hist + wave * RADIX;
+    bool skewed = false;
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r)
     {
         const uint32_t d = dig(k[r]);
-        // Rank: one LDS atomic add per key (see onesweep_rank_scatter).
-        if (FULL || wave_base + r * WAVE + lane < n)
-            rank[r] = __hip_atomic_fetch_add(&my_hist[d], 1u, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (r == 0) skewed = round_is_skewed(d);
+        rank[r] = rank_digit(my_hist, d, FULL || wave_base + r * WAVE + lane < n, skewed);
     }
     __syncthreads();
     uint32_t total = 0;

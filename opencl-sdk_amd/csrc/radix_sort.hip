@@ -13,7 +13,8 @@
 //                    at once (LDS counters, one global add per bin per block)
 //   digit_scan       exclusive scan of each place's 256 counts
 //   onesweep (x P)   per tile: ranks from per-wave digit counters in LDS
-//                    (one atomic add per key), decoupled look-back across tiles per
+//                    (an atomic add per key, or a ballot match for rounds of
+//                    mostly equal digits), decoupled look-back across tiles per
 //                    digit, tile reordered in LDS by digit and written to HBM
 //                    in contiguous digit runs.
 //

@@ -1,7 +1,8 @@
 // Building blocks of the gfx950 radix sorts, shared by the plain sort
 // (radix_sort.hip) and the segmented sort (segmented_sort.hip): the
-// look-back status words, the stable LDS-atomic ranking of 8-bit digits, and
-// the processing of one onesweep tile.
+// look-back status words, the stable ranking of 8-bit digits (LDS atomics,
+// or a wave64 ballot match for rounds of mostly equal digits), and the
+// processing of one onesweep tile.
 #pragma once
 
 #include "device_common.hpp"
@@ -21,6 +22,86 @@ template <typename S> struct status_bits
     static constexpr S FLAGS = S(3) << SHIFT;
     static constexpr S VALUE = ~FLAGS;
 };
+
+//! Lanes of the wave whose digit equals this lane's: 8 ballots, one per
+//! digit bit. m is the lane's bit b sign-extended to all ones or zero, so
+//! ballot ^ m has a 1 exactly at the lanes that differ in bit b; OR over the
+//! bits leaves 0 at the peers.
+__device__ __forceinline__ uint64_t match_digit(uint32_t d)
+{
+    // All eight ballots are taken before they are folded, so each lands in
+    // an SGPR pair of its own and the folds read ballots written a few
+    // instructions earlier: a ballot folded at once (through VCC) costs
+    // hazard wait states on gfx950 (s_nop in the code per bit: 512 -> 92 in
+    // the u32 pair pass).
+    uint32_t m[8];
+    uint64_t bal[8];
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+    {
+        m[b] = static_cast<uint32_t>(static_cast<int32_t>(d << (31 - b)) >> 31);
+        // Opaque to the optimiser, so the ballot compares m itself (one
+        // v_bfe_i32 + one v_cmp per bit, instead of re-deriving the bit with
+        // a second shift) and each half folds into one three-input op.
+        asm("" : "+v"(m[b]));
+        bal[b] = __ballot(m[b] != 0);
+    }
+    uint32_t diff_lo = 0, diff_hi = 0;
+#pragma unroll
+    for (int b = 0; b < 7; ++b)
+    {
+        diff_lo |= static_cast<uint32_t>(bal[b]) ^ m[b];
+        diff_hi |= static_cast<uint32_t>(bal[b] >> 32) ^ m[b];
+        // Keeps the accumulation a chain of diff | (bal ^ m) steps, each one
+        // v_bitop3, rather than XORs re-associated into OR trees.
+        asm("" : "+v"(diff_lo), "+v"(diff_hi));
+    }
+    // The last fold also complements: ~(diff | (bal ^ m)) is one v_bitop3
+    // per half (a separate NOT cost two instructions a row: 2% of the pass).
+    const uint32_t lo = ~(diff_lo | (static_cast<uint32_t>(bal[7]) ^ m[7]));
+    const uint32_t hi = ~(diff_hi | (static_cast<uint32_t>(bal[7] >> 32) ^ m[7]));
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+//! Lanes of a round on one digit from which a wave ranks by the ballot match
+//! instead of LDS atomics: an LDS atomic serves the lanes that hit one
+//! address one after another, so rounds of mostly equal digits (the
+//! constant-ish high bytes of skewed keys, runs of duplicates) cost about
+//! one LDS cycle a lane there, while the match costs the same for any
+//! digits.
+constexpr uint32_t SKEW_LANES = 16;
+
+//! Whether a wave ranks its onesweep tile by the match: decided once, from
+//! its first round, by the lanes sharing the first or the last lane's digit
+//! (a test every round cost 0.5% of the uniform u32 pass).
+__device__ __forceinline__ bool round_is_skewed(uint32_t d)
+{
+    const uint32_t a = __builtin_amdgcn_readfirstlane(d);
+    const uint32_t b = __builtin_amdgcn_readlane(d, WAVE - 1);
+    return __popcll(__ballot(d == a)) >= SKEW_LANES || __popcll(__ballot(d == b)) >= SKEW_LANES;
+}
+
+//! Stable rank of this lane's digit d among the wave's keys so far, and the
+//! wave's counter of d advanced past this round: the number of earlier
+//! keys of the wave with digit d (my_hist[d] before the round) plus the
+//! valid lanes below this one with digit d. Invalid lanes neither count
+//! nor get a rank. With skewed (wave-uniform) the ballot match ranks the
+//! round; otherwise one LDS atomic add per lane, which gfx950 serves in
+//! lane order for lanes that hit the same address (so equal digits rank in
+//! lane order; a wave's rounds follow in program order).
+__device__ __forceinline__ uint32_t rank_digit(uint32_t* my_hist, uint32_t d, bool valid,
+                                               bool skewed)
+{
+    if (!skewed)
+        return valid ? __hip_atomic_fetch_add(&my_hist[d], 1u, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_WORKGROUP)
+                     : 0u;
+    const uint64_t peers = match_digit(d) & __ballot(valid);
+    const uint32_t pre = count_below(peers);
+    const uint32_t base = my_hist[d];
+    if (valid && pre == 0) my_hist[d] = base + __popcll(peers);
+    return base + pre;
+}
 
 //! Digit of a key for one pass: byte shift/8 of the key's order image.
 template <typename K, int KIND, bool DESC> struct key_digit
@@ -167,24 +248,20 @@ __device__ __forceinline__ void onesweep_rank_scatter(
     const uint32_t wave_base = tile_base + wave * (WAVE * ITEMS);
 
     // Rank inside the wave: round by round, each key's rank among equal
-    // digits is the per-wave running count of its digit, taken and advanced
-    // by one LDS atomic add per key. gfx950 serves the lanes of one LDS
-    // atomic that hit the same address in lane order, so equal digits of a
-    // round rank in lane order, and a wave's rounds follow in program
-    // order: the ranking is stable without a match of the lanes' digits.
+    // digits is the per-wave running count of its digit (rank_digit).
     constexpr bool STAGED = DF::STAGED;
     uint32_t rank[ITEMS];
     uint32_t dk[STAGED ? (ITEMS + 3) / 4 : 1]; // digits of k[], four per word
     uint32_t* my_hist = s_hist + wave * RADIX;
+    bool skewed = false;
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r)
     {
         const uint32_t d = dig(k[r]);
         if constexpr (STAGED)
             dk[r / 4] = r % 4 == 0 ? d : dk[r / 4] | (d << (8 * (r % 4)));
-        if (FULL || wave_base + r * WAVE + lane < n)
-            rank[r] = __hip_atomic_fetch_add(&my_hist[d], 1u, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (r == 0) skewed = round_is_skewed(d);
+        rank[r] = rank_digit(my_hist, d, FULL || wave_base + r * WAVE + lane < n, skewed);
     }
     __syncthreads();
 
@@ -459,7 +536,7 @@ __device__ __forceinline__ void digit_bases(uint32_t* s_hist, uint32_t* s_tmp)
 
 //! One stable LDS counting pass of a block over words w[] (row r < rows of
 //! wave `wave` holds position wave * 64 * rows + r * 64 + lane) by the
-//! nb-bit digit (w >> shift) & (2^nb - 1): LDS-atomic ranks,
+//! nb-bit digit (w >> shift) & (2^nb - 1): rank_digit ranks,
 //! per-wave digit counters, an LDS reorder. Only positions below m are
 //! keys; rows of a wave that hold none are skipped, and the other positions
 //! past m take the largest digit, so they end behind every key (stability).
@@ -487,10 +564,10 @@ __device__ __forceinline__ void lds_pass(W (&w)[ITEMS], uint32_t (&v)[ITEMS], ui
             const bool valid = wave_base + r * WAVE + lane < m;
             const uint32_t d = valid ? static_cast<uint32_t>(w[r] >> shift) & mask : mask;
             dg[r / 4] = r % 4 == 0 ? d : dg[r / 4] | (d << (8 * (r % 4)));
-            // The rank among equal digits: one LDS atomic add on the wave's
-            // counter (lanes served in lane order; see onesweep_rank_scatter).
-            rank[r] = __hip_atomic_fetch_add(&my_hist[d], 1u, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_WORKGROUP);
+            // LDS atomics only: the match's registers cost the 128 x 8 and
+            // 256 x 8 shapes a wave per SIMD (the segmented graph 1.05 ->
+            // 1.10 ms), and a segment's keys are relative to its minimum.
+            rank[r] = rank_digit(my_hist, d, true, false);
         }
     __syncthreads();
     digit_bases<BLOCK>(s_hist, s_tmp);
