@@ -20,7 +20,7 @@ namespace radix {
 //! alone.
 constexpr int NOS = 11;
 constexpr int OS_BLOCK[NOS] = { 1024, 1024, 1024, 1024, 512, 512, 1024, 1024, 1024, 1024, 1024 };
-constexpr int OS_ITEMS[NOS] = { 16, 24, 28, 16, 24, 16, 20, 12, 16, 11, 20 };
+constexpr int OS_ITEMS[NOS] = { 16, 24, 26, 16, 24, 16, 20, 12, 16, 11, 20 };
 constexpr bool OS_SPLIT[NOS] = { true, true, true, false, true, true, true, true,
                                  false, false, false };
 constexpr int OS_WAVES[NOS] = { 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4 };
@@ -51,13 +51,15 @@ inline bool os_feasible(int cfg, std::size_t kbytes, bool values)
         && os_lds(cfg, kbytes, values) + OS_STATIC_LDS <= OS_LDS_LIMIT;
 }
 
-//! Untuned shape: the largest tiles that fit one workgroup per CU. Measured
-//! on MI355X, 2^28 u32 pairs: shape 6 (1024 x 20, split) 58.1 GKeys/s,
-//! shape 3 56.3, shape 0 53.4, every two-groups-per-CU shape 42.8-51.8;
-//! 2^27 u64 pairs: shape 0 (1024 x 16, split) 27.3, the rest 19.2-24.1.
+//! Untuned shape: the largest tiles that fit one workgroup per CU without
+//! spilling. Measured on MI355X, 2^28 u32 pairs, ms per pass: shape 2
+//! (1024 x 26, split) 0.962-0.966, shape 1 (1024 x 24) 0.967-0.968, shape 6
+//! (1024 x 20) 0.994-0.995, shape 0 (1024 x 16) 1.053 (1024 x 28 spilled
+//! three VGPRs: 0.964); 2^27 u64 pairs: shape 0 (1024 x 16, split) 0.746,
+//! the others that fit no faster.
 inline int os_default(std::size_t kbytes, bool values)
 {
-    return kbytes == 4 && os_feasible(6, kbytes, values) ? 6 : 0;
+    return kbytes == 4 && os_feasible(2, kbytes, values) ? 2 : 0;
 }
 
 //! One digit place. The keys and values live in two or three buffers (a =
