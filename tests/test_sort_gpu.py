@@ -153,12 +153,26 @@ def test_ragged_sizes(ctx, n):
 
 
 @pytest.mark.parametrize("case", ["all_equal", "two_values", "sorted", "reversed",
-                                  "top_bits_only", "low_bits_only", "max_keys"])
+                                  "top_bits_only", "low_bits_only", "max_keys",
+                                  "mixed_waves", "four_digits", "skew_after_first_round"])
 def test_skewed_inputs(ctx, case):
+    """Skewed keys, including inputs whose waves take both ranking paths
+    (a wave whose first round has 16 or more lanes on one digit ranks by
+    the ballot match, the others by LDS atomics): alternating runs of one
+    wave's keys (64 x 26 at the default tile) with a constant or a random
+    low byte, four low-byte values (16 lanes each, the threshold), and a
+    uniform first round followed by constant rounds."""
     n = 200003
     say("sort skewed", case, n)
     base = oracle.pcg32_u32(5, 6, n)
+    i = np.arange(n, dtype=np.uint32)
+    wave_keys = 64 * 26
     keys = {
+        "mixed_waves": np.where((i // wave_keys) % 2 == 0,
+                                (base & np.uint32(0xFFFFFF00)) | np.uint32(5), base),
+        "four_digits": (base & np.uint32(0xFFFFFF00)) | ((base >> 3) & np.uint32(3)),
+        "skew_after_first_round": np.where(i % wave_keys < 64, base,
+                                           base & np.uint32(0xFFFF0000)),
         "all_equal": np.full(n, 0xDEADBEEF, np.uint32),
         "two_values": np.where(base & 1, 3, 0xFFFFFFF0).astype(np.uint32),
         "sorted": np.sort(base),
@@ -166,7 +180,7 @@ def test_skewed_inputs(ctx, case):
         "top_bits_only": base & np.uint32(0xFF000000),
         "low_bits_only": base & np.uint32(0xFF),
         "max_keys": np.full(n, 0xFFFFFFFF, np.uint32),
-    }[case]
+    }[case].astype(np.uint32)
     vals = np.arange(n, dtype=np.uint32)
     k, v = gpu_sort(ctx, keys, vals)
     ek, ev = oracle.radix_sort(keys, vals)
