@@ -514,8 +514,12 @@ __device__ __forceinline__ void digit_bases(uint32_t* s_hist, uint32_t* s_tmp)
         }
     }
     const uint32_t inc = wave_inclusive_add(local);
-    if (tid < T && lane == WAVE - 1) s_tmp[wave] = inc;
-    __syncthreads();
+    // One wave owns every digit when the block is one wave: no exchange.
+    if constexpr (TW > 1)
+    {
+        if (tid < T && lane == WAVE - 1) s_tmp[wave] = inc;
+        __syncthreads();
+    }
     if (tid < T)
     {
         uint32_t start = inc - local;
@@ -541,8 +545,10 @@ __device__ __forceinline__ void digit_bases(uint32_t* s_hist, uint32_t* s_tmp)
 //! keys; rows of a wave that hold none are skipped, and the other positions
 //! past m take the largest digit, so they end behind every key (stability).
 //! With MOVE set, values v[] follow the words through s_v in the same
-//! reorder. The counters must be zero on entry and are zero again on exit
-//! (cleared while the reordered rows are read back).
+//! reorder. The counters must be zero on entry; each wave clears its own
+//! after reading its rows back, so they are all zero again once every wave
+//! has returned (a barrier after the call, or the next pass's first one).
+//! The reordered words stay in s_w, complete on return.
 template <typename W, int MOVE, int BLOCK, int ITEMS>
 __device__ __forceinline__ void lds_pass(W (&w)[ITEMS], uint32_t (&v)[ITEMS], uint32_t shift,
                                          uint32_t nb, uint32_t m, uint32_t rows,
@@ -587,8 +593,11 @@ __device__ __forceinline__ void lds_pass(W (&w)[ITEMS], uint32_t (&v)[ITEMS], ui
             w[r] = s_w[wave_base + r * WAVE + lane];
             if constexpr (MOVE == 1) v[r] = s_v[wave_base + r * WAVE + lane];
         }
-    for (uint32_t i = tid; i < NW * RADIX; i += BLOCK) s_hist[i] = 0;
-    __syncthreads();
+    // Each wave clears its own counters: nothing reads them before the next
+    // pass's first barrier, which also orders these read-backs before the
+    // next reorder, so the pass needs no closing barrier.
+#pragma unroll
+    for (int q = 0; q < static_cast<int>(RADIX / WAVE); ++q) my_hist[q * WAVE + lane] = 0;
 }
 
 //! Stable LSD passes of lds_pass over bits [lo, lo + span) of the words:
