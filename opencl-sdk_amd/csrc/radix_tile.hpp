@@ -638,11 +638,14 @@ constexpr std::size_t lds_segment_bytes_per_key(std::size_t kbytes, bool values)
 //! bytes a key and no value, and each value is fetched once at the end by
 //! position (and each key too when the range passes 48 bits). Then one
 //! contiguous write back. LDS (s_hist, s_buf, s_tmp) may be reused after
-//! the caller's next barrier.
+//! the caller's next barrier. A workgroup sorting segments one after
+//! another passes first for its first one only and a barrier between
+//! them: each call leaves the digit counters zero and the range words
+//! reset for the next (the first call sets both up itself).
 template <typename K, int KIND, bool DESC, bool VALUES, int BLOCK, int ITEMS>
 __device__ __forceinline__ void lds_sort_segment(
     K* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t m, uint32_t nbmax,
-    uint32_t* s_hist, void* s_buf, uint32_t* s_tmp)
+    uint32_t* s_hist, void* s_buf, uint32_t* s_tmp, bool first)
 {
     constexpr bool WIDE = sizeof(K) == 8;
     constexpr uint32_t TILE = BLOCK * ITEMS;
@@ -660,11 +663,16 @@ __device__ __forceinline__ void lds_sort_segment(
 
     using U = typename std::conditional<WIDE, unsigned long long, unsigned int>::type;
     __shared__ U s_min, s_max;
-    if (tid == 0)
-    {
-        s_min = ~U(0);
-        s_max = 0;
-    }
+    // Resets the range words for the next segment; called once every
+    // thread has read them and passed a barrier since.
+    auto reset_range = [&] {
+        if (tid == 0)
+        {
+            s_min = ~U(0);
+            s_max = 0;
+        }
+    };
+    if (first) reset_range();
     const uint32_t live = m > wave_base ? min(rows, (m - wave_base + WAVE - 1) / WAVE) : 0u;
     // Loads first, unconditionally (positions past m read key m - 1), so
     // they are all in flight together: no load waits on a lane or row test
@@ -699,8 +707,11 @@ __device__ __forceinline__ void lds_sort_segment(
         lo = min(lo, static_cast<U>(__shfl_xor(lo, o, WAVE)));
         hi = max(hi, static_cast<U>(__shfl_xor(hi, o, WAVE)));
     }
-    for (uint32_t i = tid; i < (BLOCK / WAVE) * RADIX; i += BLOCK) s_hist[i] = 0;
-    __syncthreads(); // s_min / s_max initialised, digit counters zero
+    if (first)
+    {
+        for (uint32_t i = tid; i < (BLOCK / WAVE) * RADIX; i += BLOCK) s_hist[i] = 0;
+        __syncthreads(); // s_min / s_max initialised, digit counters zero
+    }
     if (lane == 0 && wave_base < m)
     {
         atomicMin(&s_min, lo);
@@ -711,6 +722,7 @@ __device__ __forceinline__ void lds_sort_segment(
     if (range == 0)
     {
         __syncthreads(); // everyone has read s_min / s_max
+        reset_range();
         return;
     }
     // The same in every lane (read from LDS, which the compiler cannot
@@ -781,6 +793,7 @@ __device__ __forceinline__ void lds_sort_segment(
                 vals[pos] = v[r];
             }
         }
+        reset_range();
         return; // no LDS use after the last pass's barrier
 
     }
@@ -820,6 +833,7 @@ __device__ __forceinline__ void lds_sort_segment(
                 vals[pos] = v[r];
             }
         }
+        reset_range();
         return;
     }
     constexpr int MOVE = 0; // 32-bit keys without values
@@ -838,6 +852,7 @@ __device__ __forceinline__ void lds_sort_segment(
             if constexpr (VALUES) vals[pos] = v[r];
         }
     }
+    reset_range(); // the passes' barriers follow every read of the range
 }
 
 } // namespace radix

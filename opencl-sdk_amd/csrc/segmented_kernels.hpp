@@ -102,7 +102,8 @@ __global__ __launch_bounds__(BLOCK) void lds_class_kernel(
         const uint64_t b = e.x | (static_cast<uint64_t>(e.y) << 32);
         const uint32_t nb = min(max(e.w, 1u), static_cast<uint32_t>(MAX_DIGIT_BITS));
         lds_sort_segment<K, KIND, DESC, VALUES, BLOCK, ITEMS>(
-            keys + b, VALUES ? vals + b : nullptr, e.z, nb, s_hist, s_buf, s_tmp);
+            keys + b, VALUES ? vals + b : nullptr, e.z, nb, s_hist, s_buf, s_tmp,
+            i == blockIdx.x);
         __syncthreads(); // the next segment reuses the LDS
     }
 }
